@@ -1,0 +1,142 @@
+/* rsim: the MI355X (gfx950) batched RocketSim V3 decision stepper — C ABI of librsim.so.
+ *
+ * Drop-in for the reference crate's device API (crates/rocketsim-cuda/src/device.rs:15-172, SimDevice and its buffers).
+ * Conventions of every entry point:
+ *   - returns an int status: RSIM_OK (0) or a negative RSIM_ERR_*; never throws, never aborts on misuse (the reference
+ *     panics with assert! where this returns RSIM_ERR_INVALID / RSIM_ERR_SHORT);
+ *   - handles are opaque and owned by the caller, freed explicitly (rsim_*_free, rsim_close);
+ *   - host memory is caller-owned; `ArenaState`, `StaticData` and `Controls` are passed as `void*` to buffers in the
+ *     reference's word format (cuda/sim/arena_state.cuh:38-342: ArenaState 11,136 B, StaticData 1,271,580 B, Controls
+ *     32 B = throttle, steer, pitch, yaw, roll as f32, then jump, boost, handbrake as u32 0/1);
+ *   - all device work of one rsim_device goes to its one stream, in call order: rsim_sim8 and rsim_run are asynchronous,
+ *     everything that returns host data synchronises first. A device handle is not thread-safe.
+ * rsim_last_error() describes the last failure of the calling thread. */
+#ifndef RSIM_H
+#define RSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------------------------------- */
+#define RSIM_OK 0
+#define RSIM_ERR_INVALID (-1)   /* bad argument: null pointer, wrong size, block outside 1..RSIM_MAX_BLOCK ... */
+#define RSIM_ERR_SHORT (-2)     /* a buffer is shorter than the call needs (device.rs:158, :163) */
+#define RSIM_ERR_HIP (-3)       /* a HIP runtime call failed (DriverError in the reference) */
+#define RSIM_ERR_NO_DEVICE (-4) /* no such GPU, or no GPU at all */
+
+/* PhaseStatus of the per-arena status words (cuda/sim/phase.cuh:18). */
+#define RSIM_PHASE_OK 0
+#define RSIM_PHASE_UNIMPLEMENTED (-1)
+#define RSIM_PHASE_BAD_INDEX (-2)
+
+/* ---- the decision and its result word (device.rs:15-28; cuda/sim/arena_step.cuh:16-19) ----------------------------- */
+#define RSIM_TICKS_PER_DECISION 8
+#define RSIM_SIM8_SCORED_MASK 0xFFu         /* the tick 1..8 after which the ball was first scored, 0 for none */
+#define RSIM_SIM8_UNIMPLEMENTED 0x80000000u /* a tick up to then reached a shape pair V3's port does not implement */
+#define RSIM_SIM8_OVERFLOW 0x40000000u      /* an overflow counter of the state written back is nonzero */
+#define RSIM_SIM8_BLOCK 64                  /* work-group size in production: one wave64 per AoSoA-64 block */
+#define RSIM_MAX_BLOCK 256                  /* largest work-group size the tick kernels accept */
+#define RSIM_NUM_PHASES 10                  /* P0, P1, P2, P3a, P3b, P4, P5, P6, P7, P8 (phase.cuh:17-46) */
+#define RSIM_AOSOA_WIDTH 64                 /* arenas per AoSoA block */
+
+/* ---- layout queries (crates/rocketsim-cuda/csrc/phase.cpp:18-22 lf_*; src/aosoa.rs:1-38) --------------------------- */
+uint64_t rsim_arena_words(void);  /* 2,784 u32 words per ArenaState */
+uint64_t rsim_tick_words(void);   /* 2,567: the words that persist across a tick boundary; the rest is zero there */
+uint64_t rsim_static_words(void); /* 317,895 u32 words per StaticData */
+/* u32 words of an AoSoA-64 buffer of n arenas: ceil(n / 64) blocks of 2,784 x 64 words. */
+uint64_t rsim_aosoa_words(uint32_t n);
+/* Position of word w of arena a: (a / 64) * 2784 * 64 + w * 64 + a % 64. */
+uint64_t rsim_aosoa_index(uint32_t a, uint32_t w);
+/* Host-side pack / unpack between n ArenaStates and an AoSoA-64 buffer (aosoa.rs:17-38); pack zero-fills padding lanes. */
+int rsim_aosoa_pack(const void* states, uint32_t n, uint32_t* aosoa);
+int rsim_aosoa_unpack(const uint32_t* aosoa, uint32_t n, void* states);
+
+/* ---- the device (device.rs:30-67) ------------------------------------------------------------------------------------ */
+typedef struct rsim_device rsim_device;
+typedef struct rsim_world rsim_world;   /* StaticData on the device (DeviceStatic, device.rs:47-50) */
+typedef struct rsim_states rsim_states; /* n arenas in one AoSoA-64 device buffer (DeviceStates, device.rs:41-45) */
+typedef struct rsim_words rsim_words;   /* a plain u32 device buffer (controls, results, traces: CudaSlice<u32>) */
+
+/* SimDevice::new (device.rs:53-67): opens GPU `ordinal`, creates its stream (default priority) and timing events. */
+int rsim_open(int ordinal, rsim_device** dev);
+int rsim_close(rsim_device* dev);
+/* The device's HIP stream (a hipStream_t) and its ordinal, for callers that order their own work against it. */
+void* rsim_stream(rsim_device* dev);
+
+/* upload_static (device.rs:69-71): one StaticData (rsim_static_words() words). */
+int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** world);
+int rsim_world_free(rsim_world* world);
+
+/* upload_states / alloc_states (device.rs:73-80): n ArenaStates packed to AoSoA-64 on the host and uploaded; alloc zeroes. */
+int rsim_upload_states(rsim_device* dev, const void* states, uint32_t n, rsim_states** out);
+int rsim_alloc_states(rsim_device* dev, uint32_t n, rsim_states** out);
+/* Overwrites the n states of an existing buffer (same n) without reallocating. */
+int rsim_write_states(rsim_device* dev, rsim_states* dst, const void* states);
+/* Device-to-device copy of a whole state buffer (same n): the restore between timed runs (tests/speed.rs:65-75). */
+int rsim_copy_states(rsim_device* dev, rsim_states* dst, const rsim_states* src);
+int rsim_states_free(rsim_states* states);
+uint32_t rsim_states_count(const rsim_states* states);
+void* rsim_states_device_ptr(rsim_states* states);
+/* download_words / download_states (device.rs:82-88): synchronous. */
+int rsim_download_words(rsim_device* dev, const rsim_states* states, uint32_t* aosoa_out);
+int rsim_download_states(rsim_device* dev, const rsim_states* states, void* states_out);
+
+/* Plain u32 device buffers. upload_controls (device.rs:143-149) is rsim_upload_controls: Controls[n][2] (16 words per arena). */
+int rsim_alloc_words(rsim_device* dev, uint64_t words, rsim_words** out);
+int rsim_upload_words(rsim_device* dev, const uint32_t* host, uint64_t words, rsim_words** out);
+int rsim_write_words(rsim_device* dev, rsim_words* dst, const uint32_t* host, uint64_t words);
+int rsim_read_words(rsim_device* dev, const rsim_words* src, uint32_t* host, uint64_t words);
+int rsim_upload_controls(rsim_device* dev, const void* controls, uint32_t n, rsim_words** out);
+int rsim_words_free(rsim_words* buf);
+uint64_t rsim_words_count(const rsim_words* buf);
+void* rsim_words_device_ptr(rsim_words* buf);
+
+/* sim8 (device.rs:151-168): one decision (8 ticks) of every arena of `states`, in place, asynchronously on the device's
+ * stream. controls: >= n * 16 words (rsim_upload_controls). result: >= n words, result[a] the RSIM_SIM8_* word of arena a.
+ * block: work-group size, 1..RSIM_MAX_BLOCK (RSIM_SIM8_BLOCK in production); results do not depend on it. trace: null,
+ * or >= 8 * rsim_aosoa_words(n) words that receive the state after tick t in slab t - 1 (ticks after a goal are not
+ * written). */
+int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
+              uint32_t block, rsim_words* trace);
+
+/* The debug and parity kernels (device.rs:90-141), synchronous. status_out: n ints (PhaseStatus per arena) or null. */
+int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out);      /* copy-in, copy-out */
+int rsim_identity_tick(rsim_device* dev, const rsim_states* in, rsim_states* out); /* tick-boundary copy: tail zeroed */
+int rsim_phase(rsim_device* dev, int k, rsim_states* states, const rsim_world* world, int32_t* status_out);
+int rsim_step_tick(rsim_device* dev, rsim_states* states, const rsim_world* world, int32_t* status_out);
+
+/* Timing on the device's stream: HIP events recorded before and after the work queued between begin and end. */
+int rsim_timer_begin(rsim_device* dev);
+int rsim_timer_end(rsim_device* dev, float* ms); /* synchronises; ms = elapsed device time */
+int rsim_sync(rsim_device* dev);
+
+/* The message of the calling thread's last failed call ("" if none). */
+const char* rsim_last_error(void);
+
+/* ---- synthetic workloads (host only; the stand-in for RocketSim V3's exports, crates/rocketsim-cuda/src/dump.rs) ---- */
+#define RSIM_NUM_PRESETS 7 /* Octane, Dominus, Plank, Breakout, Hybrid, Merc, Psyclops (v3dump/src/scenario.rs:21-29) */
+#define RSIM_SCEN_KICKOFF 0   /* one of the 5 kickoff spots, both cars mirrored */
+#define RSIM_SCEN_OPEN 1      /* open play: cars and ball anywhere, random pads */
+#define RSIM_SCEN_PINCH 2     /* the ball within 200 uu of a wall or fillet, a car driving into it */
+#define RSIM_SCEN_HEADON 3    /* two cars closing at 1500-2300 uu/s from 300-1200 uu */
+#define RSIM_SCEN_GOALFRAME 4 /* the ball within 150 uu of a post or the crossbar, moving at it */
+#define RSIM_SCEN_CONTACT 5   /* 40% pinch, 30% head-on, 30% goal frame */
+#define RSIM_SCEN_MIXED 6     /* 1:1:1 kickoff, open play, contact */
+
+/* The StaticData of one world: the procedural Soccar arena, car 0 with preset0 and car 1 with preset1, and the mutators
+ * (Soccar defaults: gravity_z -650, boost_accel_scale 1, ball_radius 91.25). */
+int rsim_build_world(int preset0, int preset1, float gravity_z, float boost_accel_scale, float ball_radius, void* world_out);
+/* Arenas first..first+n-1 of a seeded workload: arena i depends only on (seed, i), so any subrange is reproducible. */
+int rsim_sample_arenas(const void* world, int scenario, uint64_t seed, uint32_t first, uint32_t n, void* states_out);
+/* The controls of one decision for arenas first..first+n-1 (Controls[n][2]): axes uniform in {-1, 0, 1}, jump / boost /
+ * handbrake Bernoulli(0.3). */
+int rsim_sample_controls(uint64_t seed, uint32_t decision, uint32_t first, uint32_t n, void* controls_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSIM_H */

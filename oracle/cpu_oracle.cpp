@@ -1,0 +1,49 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle. The product never links this file.
+//
+// The CPU build of the single-source tick (rocketsim-cuda_amd/csrc/core), behind the C ABI of the reference's host build of
+// the same path (crates/rocketsim-cuda/csrc/phase.cpp:8-40: rscuda_phase, rscuda_step_tick, rscuda_step_ticks, lf_*). Tests
+// pin it bit for bit to the reference's own C++ build (oracle/_ref) on golden vectors, then check the gfx950 kernels against it.
+// Built by oracle/Makefile with g++ -O2 -ffp-contract=off (no FMA target).
+#include "core/rs_tick.h"
+
+using namespace rs;
+
+extern "C" {
+// phase.cpp:10 rscuda_phase
+int rsim_cpu_phase(int k, ArenaState* s, const StaticData* w) { return run_phase(k, *s, *w); }
+// phase.cpp:12 rscuda_step_tick
+int rsim_cpu_step_tick(ArenaState* s, const StaticData* w) { return step_tick(*s, *w); }
+// phase.cpp:15-17 rscuda_step_ticks: n ticks with controls[0..1] set before each; *s ends as the goal state (scored tick in
+// *scored_tick, 0 for none) or the state after tick n; trace (null or n states): trace[t-1] = the state after tick t up to the
+// goal. Returns 0 (PHASE_OK) or -1 (PHASE_UNIMPLEMENTED).
+int rsim_cpu_step_ticks(ArenaState* s, const StaticData* w, const Controls* controls, uint32_t n, uint32_t* scored_tick,
+                        ArenaState* trace) {
+  uint32_t r = run_decision(*s, *w, controls, n, [trace](uint32_t t, const ArenaState& st) {
+    if (trace) trace[t - 1] = st;
+  });
+  *scored_tick = r & SIM8_SCORED_MASK;
+  return (r & SIM8_UNIMPLEMENTED) ? PHASE_UNIMPLEMENTED : PHASE_OK;
+}
+// The result word of one decision for each of n arenas (AoS, in place): the CPU baseline of the 8-tick decision.
+void rsim_cpu_sim8_aos(ArenaState* states, const StaticData* w, const Controls* controls, uint32_t* result, uint32_t n) {
+  for (uint32_t a = 0; a < n; ++a) result[a] = run_decision(states[a], *w, controls + 2 * (size_t)a, TICKS_PER_DECISION, NoTrace{});
+}
+uint64_t rsim_cpu_arena_words() { return ARENA_WORDS; }
+uint64_t rsim_cpu_tick_words() { return TICK_WORDS; }
+uint64_t rsim_cpu_static_words() { return STATIC_WORDS; }
+
+// The correctly rounded functions over inputs, for the libm checks: fn 0 sin, 1 cos, 2 tan over the bit patterns base..base+n;
+// binary: op 0 atan2f(a[i], b[i]), 1 powf(a[i], b[i]); f64: op 0 atan2_64, 1 hypot64.
+void rsim_cpu_libm_unary(int fn, uint64_t base, uint64_t n, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    float x = ffrom((uint32_t)(base + i));
+    out[i] = fbits(fn == 0 ? crm::sinf_cr(x) : fn == 1 ? crm::cosf_cr(x) : crm::tanf_cr(x));
+  }
+}
+void rsim_cpu_libm_binary(int op, const float* a, const float* b, uint64_t n, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = fbits(op == 0 ? crm::atan2f_cr(a[i], b[i]) : crm::powf_cr(a[i], b[i]));
+}
+void rsim_cpu_libm_f64(int op, const double* a, const double* b, uint64_t n, double* out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = op == 0 ? crm::atan2_64(a[i], b[i]) : crm::hypot64(a[i], b[i]);
+}
+}

@@ -1,0 +1,270 @@
+// Phase P3a, the broadphase: swept AABBs of the three dynamic bodies, Bullet's 3-axis sweep and prune with u16 edges and
+// incremental edge sorts that add and remove overlapping pairs, the pair array (append; swap-remove), and the dynamic
+// ray-grid cell lists. Reference: cuda/sim/collision_world.cuh, grid_broadphase.cuh, axis_sweep.cuh,
+// overlapping_pair_cache.cuh, collision_dispatcher_release.cuh.
+#pragma once
+#include "rs_ray.h"
+
+namespace rs {
+
+// ---- shape AABBs (collision_world.cuh:10-31, narrowphase.cuh:75-118) ---------------------------------------------------------
+RS_FN Aabb sphere_box(float radius, V3 c) {
+  V3 e = splat(radius + k::SPHERE_RADIUS_MARGIN);
+  return Aabb{c - e, c + e};
+}
+RS_FN Aabb compound_box(const BodyConst& b, const Aff& t) {  // compound_shape.rs get_aabb
+  V3 half = 0.5f * (b.compound_local_aabb.max - b.compound_local_aabb.min);
+  V3 ctr = 0.5f * (b.compound_local_aabb.max + b.compound_local_aabb.min);
+  M3 ab = mabs(t.m);
+  V3 c = xform_point(t, ctr), e = ab * half;
+  return Aabb{c - e, c + e};
+}
+RS_FN Aabb box_from_half_extents(V3 half, float margin, const Aff& t) {  // shared/aabb.rs from_half_extents_transform
+  V3 hm = add_s(half, margin);
+  V3 e = mabs(t.m) * hm;
+  return Aabb{t.t - e, t.t + e};
+}
+
+// ---- the pair array (overlapping_pair_cache.cuh) -----------------------------------------------------------------------------
+RS_FN int find_pair(const ArenaState& s, const StaticData& w, uint32_t id0, uint32_t id1) {
+  for (uint32_t i = 0; i < s.n_pairs; ++i)
+    if (w.proxies[s.pairs[i].proxy0].unique_id == id0 && w.proxies[s.pairs[i].proxy1].unique_id == id1) return (int)i;
+  return -1;
+}
+RS_FN void add_pair(ArenaState& s, const StaticData& w, uint32_t h0, uint32_t h1) {
+  const ProxyConst &p0 = w.proxies[h0], &p1 = w.proxies[h1];
+  if (!((p0.filter_group & p1.filter_mask) != 0 && (p1.filter_group & p0.filter_mask) != 0)) return;
+  bool lo0 = p0.unique_id <= p1.unique_id;
+  uint32_t id0 = lo0 ? p0.unique_id : p1.unique_id, id1 = lo0 ? p1.unique_id : p0.unique_id;
+  uint32_t a = lo0 ? h0 : h1, b = lo0 ? h1 : h0;
+  if (find_pair(s, w, id0, id1) >= 0) return;
+  if (s.n_pairs >= (uint32_t)MAX_PAIRS) {
+    s.overflow[OVF_PAIRS] += 1;
+    return;
+  }
+  s.pairs[s.n_pairs++] = Pair{a, b};
+}
+// collision_dispatcher.rs release_manifold (collision_dispatcher_release.cuh:10-39): the manifold of the unordered body
+// pair leaves the pool by swap-remove; the active list drops it and renumbers the moved one.
+RS_FN int find_manifold(const ArenaState& s, uint32_t b0, uint32_t b1) {
+  uint32_t lo = b0 < b1 ? b0 : b1, hi = b0 < b1 ? b1 : b0;
+  for (uint32_t i = 0; i < s.n_manifolds; ++i) {
+    uint32_t a = s.manifolds[i].body0, b = s.manifolds[i].body1;
+    if ((a < b ? a : b) == lo && (a < b ? b : a) == hi) return (int)i;
+  }
+  return -1;
+}
+RS_FN void release_manifold(ArenaState& s, uint32_t b0, uint32_t b1) {
+  int f = find_manifold(s, b0, b1);
+  if (f < 0) return;
+  uint32_t idx = (uint32_t)f, last = s.n_manifolds - 1;
+  uint32_t n = 0;
+  for (uint32_t r = 0; r < s.n_active; ++r)
+    if (s.active[r] != idx) s.active[n++] = s.active[r];
+  for (uint32_t r = n; r < s.n_active; ++r) s.active[r] = 0;
+  s.n_active = n;
+  if (idx != last) {
+    s.manifolds[idx] = s.manifolds[last];
+    for (uint32_t r = 0; r < s.n_active; ++r)
+      if (s.active[r] == last) s.active[r] = idx;
+  }
+  s.manifolds[last] = Manifold{};
+  s.n_manifolds = last;
+}
+RS_FN void remove_pair(ArenaState& s, const StaticData& w, uint32_t ha, uint32_t hb) {  // axis_sweep.cuh:39-46
+  uint32_t ua = w.proxies[ha].unique_id, ub = w.proxies[hb].unique_id;
+  int i = find_pair(s, w, ua < ub ? ua : ub, ua < ub ? ub : ua);
+  if (i < 0) return;
+  Pair p = s.pairs[i];
+  uint32_t last = s.n_pairs - 1;
+  if ((uint32_t)i != last) s.pairs[i] = s.pairs[last];
+  s.pairs[last] = Pair{0, 0};
+  s.n_pairs -= 1;
+  release_manifold(s, w.proxies[p.proxy0].client_body, w.proxies[p.proxy1].client_body);
+}
+
+// ---- sweep and prune (axis_sweep.cuh) --------------------------------------------------------------------------------------
+RS_FN bool edge_max(Edge e) { return (e.pos & 1u) != 0; }
+RS_FN bool overlap_2d(const ArenaState& s, uint32_t ha, uint32_t hb, int axis) {  // :28-34, touching counts
+  int a1 = (1 << axis) & 3, a2 = (1 << a1) & 3;
+  const HandleEdges &a = s.handle_edges[ha], &b = s.handle_edges[hb];
+  return !(a.max[a1] < b.min[a1] || b.max[a1] < a.min[a1] || a.max[a2] < b.min[a2] || b.max[a2] < a.min[a2]);
+}
+// The four incremental sorts (axis_sweep.cuh:49-127): the moved edge bubbles past its neighbours, each passed neighbour's
+// edge index is adjusted, and a min edge passing a max edge (or the reverse) adds or removes the pair per direction.
+RS_FN void sort_min_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
+  Edge* e = s.edges[ax];
+  uint32_t mv = e[i].handle;
+  while (i > 0 && e[i].pos < e[i - 1].pos) {
+    Edge prev = e[i - 1];
+    if (edge_max(prev)) {
+      if (overlap_2d(s, mv, prev.handle, ax)) add_pair(s, w, mv, prev.handle);
+      s.handle_edges[prev.handle].max[ax] += 1;
+    } else {
+      s.handle_edges[prev.handle].min[ax] += 1;
+    }
+    s.handle_edges[mv].min[ax] -= 1;
+    e[i - 1] = e[i];
+    e[i] = prev;
+    --i;
+  }
+}
+RS_FN void sort_max_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
+  Edge* e = s.edges[ax];
+  uint32_t mv = e[i].handle;
+  while (i + 1 < (uint32_t)NUM_EDGES && e[i + 1].handle != 0 && e[i].pos >= e[i + 1].pos) {
+    Edge next = e[i + 1];
+    if (!edge_max(next)) {
+      if (overlap_2d(s, mv, next.handle, ax)) add_pair(s, w, mv, next.handle);
+      s.handle_edges[next.handle].min[ax] -= 1;
+    } else {
+      s.handle_edges[next.handle].max[ax] -= 1;
+    }
+    s.handle_edges[mv].max[ax] += 1;
+    e[i + 1] = e[i];
+    e[i] = next;
+    ++i;
+  }
+}
+RS_FN void sort_min_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
+  Edge* e = s.edges[ax];
+  uint32_t mv = e[i].handle;
+  while (i + 1 < (uint32_t)NUM_EDGES && e[i + 1].handle != 0 && e[i].pos >= e[i + 1].pos) {
+    Edge next = e[i + 1];
+    if (edge_max(next)) {
+      if (overlap_2d(s, mv, next.handle, ax)) remove_pair(s, w, mv, next.handle);
+      s.handle_edges[next.handle].max[ax] -= 1;
+    } else {
+      s.handle_edges[next.handle].min[ax] -= 1;
+    }
+    s.handle_edges[mv].min[ax] += 1;
+    e[i + 1] = e[i];
+    e[i] = next;
+    ++i;
+  }
+}
+RS_FN void sort_max_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
+  Edge* e = s.edges[ax];
+  uint32_t mv = e[i].handle;
+  while (i > 0 && e[i].pos < e[i - 1].pos) {
+    Edge prev = e[i - 1];
+    if (!edge_max(prev)) {
+      if (overlap_2d(s, mv, prev.handle, ax)) remove_pair(s, w, mv, prev.handle);
+      s.handle_edges[prev.handle].min[ax] += 1;
+    } else {
+      s.handle_edges[prev.handle].max[ax] += 1;
+    }
+    s.handle_edges[mv].max[ax] -= 1;
+    e[i - 1] = e[i];
+    e[i] = prev;
+    --i;
+  }
+}
+RS_FN void sap_quantize(const StaticData& w, V3 p, uint16_t is_max, uint16_t out[3]) {  // axis_sweep.cuh:16-24
+  V3 sc = (p - w.sap_world_min) * w.sap_quantize;
+  V3 cl = vselect(cmpge(sc, splat(0.0f)), vmin(sc, splat((float)w.sap_sentinel)), splat(0.0f));
+  out[0] = (uint16_t)(((uint16_t)f2i_rs(cl.x) & w.sap_handle_mask) | is_max);
+  out[1] = (uint16_t)(((uint16_t)f2i_rs(cl.y) & w.sap_handle_mask) | is_max);
+  out[2] = (uint16_t)(((uint16_t)f2i_rs(cl.z) & w.sap_handle_mask) | is_max);
+}
+RS_FN void update_axis(ArenaState& s, const StaticData& w, int ax, uint32_t h, uint16_t qmin, uint16_t qmax) {
+  Edge* e = s.edges[ax];
+  uint32_t mn = s.handle_edges[h].min[ax], mx = s.handle_edges[h].max[ax];
+  int32_t dmin = (int32_t)qmin - (int32_t)e[mn].pos, dmax = (int32_t)qmax - (int32_t)e[mx].pos;
+  if (dmin == 0 && dmax == 0) return;
+  e[mn].pos = qmin;
+  e[mx].pos = qmax;
+  if (dmin < 0) sort_min_down(s, w, ax, mn);  // expand first, then shrink
+  if (dmax > 0) sort_max_up(s, w, ax, mx);
+  if (dmin > 0) sort_min_up(s, w, ax, mn);
+  if (dmax < 0) sort_max_down(s, w, ax, mx);
+}
+
+// ---- dynamic ray-grid cells (grid_broadphase.cuh) ----------------------------------------------------------------------------
+RS_FN void cell_indices(const StaticData& w, V3 p, uint32_t out[3]) {
+  V3 f = (p - w.grid_min) / w.grid_cell_size;
+  int32_t v[3] = {f2i_rs(f.x), f2i_rs(f.y), f2i_rs(f.z)};
+  for (int a = 0; a < 3; ++a) {
+    uint32_t m = (uint32_t)(v[a] > 0 ? v[a] : 0), top = w.grid_num_cells[a] - 1;
+    out[a] = m < top ? m : top;
+  }
+}
+RS_FN uint32_t cell_index(const StaticData& w, const uint32_t i[3]) {
+  return i[0] * w.grid_num_cells[1] * w.grid_num_cells[2] + i[1] * w.grid_num_cells[2] + i[2];
+}
+RS_FN void dyn_cell_push(ArenaState& s, uint32_t cell, uint32_t h) {  // after the cell's last entry
+  if (s.n_dyn_cells >= (uint32_t)MAX_DYN_CELLS) {
+    s.overflow[OVF_DYN_CELLS] += 1;
+    return;
+  }
+  uint32_t at = 0;
+  while (at < s.n_dyn_cells && s.dyn_cells[at].cell <= cell) ++at;
+  for (uint32_t i = s.n_dyn_cells; i > at; --i) s.dyn_cells[i] = s.dyn_cells[i - 1];
+  s.dyn_cells[at] = DynCell{cell, h};
+  s.n_dyn_cells += 1;
+}
+RS_FN void dyn_cell_remove(ArenaState& s, uint32_t cell, uint32_t h) {  // the cell's first match
+  for (uint32_t at = 0; at < s.n_dyn_cells; ++at) {
+    if (s.dyn_cells[at].cell == cell && s.dyn_cells[at].handle == h) {
+      for (uint32_t i = at; i + 1 < s.n_dyn_cells; ++i) s.dyn_cells[i] = s.dyn_cells[i + 1];
+      s.dyn_cells[s.n_dyn_cells - 1] = DynCell{0, 0};
+      s.n_dyn_cells -= 1;
+      return;
+    }
+  }
+}
+RS_FN void update_cells(ArenaState& s, const StaticData& w, bool add, uint32_t h, const uint32_t idx[3]) {  // the 3x3x3 block
+  uint32_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = (idx[a] > 1 ? idx[a] : 1) - 1;
+    uint32_t top = w.grid_num_cells[a] - 1, p = idx[a] + 1;
+    hi[a] = top < p ? top : p;
+  }
+  for (uint32_t i = lo[0]; i <= hi[0]; ++i)
+    for (uint32_t j = lo[1]; j <= hi[1]; ++j)
+      for (uint32_t kk = lo[2]; kk <= hi[2]; ++kk) {
+        uint32_t c3[3] = {i, j, kk};
+        uint32_t cell = cell_index(w, c3);
+        if (add) dyn_cell_push(s, cell, h);
+        else dyn_cell_remove(s, cell, h);
+      }
+}
+RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb box) {  // grid_broadphase.cuh:68-84
+  uint32_t h = w.dyn_handle[d];
+  s.dyn_proxy[d].aabb = box;
+  uint16_t qmin[3], qmax[3];
+  sap_quantize(w, box.min, 0, qmin);
+  sap_quantize(w, box.max, 1, qmax);
+  for (int ax = 0; ax < 3; ++ax) update_axis(s, w, ax, h, qmin[ax], qmax[ax]);
+  uint32_t old = s.dyn_proxy[d].cell_idx, ni[3];
+  cell_indices(w, box.min, ni);
+  uint32_t nidx = cell_index(w, ni);
+  s.dyn_proxy[d].cell_idx = nidx;
+  if (nidx != old) {
+    update_cells(s, w, false, h, s.dyn_proxy[d].indices);
+    for (int a = 0; a < 3; ++a) s.dyn_proxy[d].indices[a] = ni[a];
+    update_cells(s, w, true, h, ni);
+  }
+}
+
+// P3a (collision_world.cuh:34-50): the dynamic bodies' swept boxes (current and predicted transform, each grown by the
+// contact breaking threshold) into the broadphase.
+RS_FN void phase_p3a(ArenaState& s, const StaticData& w) {
+  const V3 cbt = splat(k::CONTACT_BREAKING_THRESHOLD);
+  for (uint32_t d = 0; d < (uint32_t)NUM_DYN; ++d) {
+    const BodyConst& bc = w.bodies[w.dyn_body[d]];
+    const Body& b = s.bodies[d];
+    if (b.collision_flags & k::CF_STATIC) continue;
+    Aabb a = bc.shape_kind == k::SHAPE_SPHERE ? sphere_box(bc.sphere_margin, b.world.t) : compound_box(bc, b.world);
+    a.min -= cbt;
+    a.max += cbt;
+    Aabb a2 = bc.shape_kind == k::SHAPE_SPHERE ? sphere_box(bc.sphere_margin, b.interp_world.t) : compound_box(bc, b.interp_world);
+    a2.min -= cbt;
+    a2.max += cbt;
+    a.min = vmin(a.min, a2.min);
+    a.max = vmax(a.max, a2.max);
+    grid_set_aabb(s, w, d, a);
+  }
+}
+
+}  // namespace rs

@@ -1,0 +1,42 @@
+// Build policy of the single-source stepper core. The same headers compile with hipcc for gfx950 (the stepper kernels) and
+// with g++ for the host (the world builder, and the CPU build the tests check against).
+//
+// Float policy (what makes the two builds agree bit for bit): every float and double operation is one IEEE operation with
+// round-to-nearest-even, written as a plain C++ operator. Both builds must compile with -ffp-contract=off (no a*b+c fusion),
+// without fast-math, and with fp32 denormals kept (gfx950: -fno-gpu-flush-denormals-to-zero); HIP's float `/` and sqrt are
+// its correctly rounded sequences (the default -fhip-fp32-correctly-rounded-divide-sqrt). The host build must have no FMA
+// target (x86-64 without -mfma/-march=native); rs_defs.h refuses to compile otherwise.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RS_HD __host__ __device__
+#define RS_FN __host__ __device__ inline
+#define RS_NOINL __host__ __device__ __attribute__((noinline))
+#define RS_TABLE static __device__ __constant__ const
+#else
+#define RS_HD
+#define RS_FN inline
+#define RS_NOINL inline
+#define RS_TABLE static const
+#endif
+
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
+#if !defined(__HIP_DEVICE_COMPILE__) && (defined(__FMA__) || defined(__FP_FAST_FMAF))
+#error "the host build of the stepper core must not target FMA (it would allow fused multiply-adds)"
+#endif
+
+namespace rs {
+
+RS_FN uint32_t fbits(float a) { return __builtin_bit_cast(uint32_t, a); }
+RS_FN float ffrom(uint32_t u) { return __builtin_bit_cast(float, u); }
+RS_FN uint64_t dbits(double a) { return __builtin_bit_cast(uint64_t, a); }
+RS_FN double dfrom(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+}  // namespace rs
