@@ -1,0 +1,172 @@
+"""Benchmark of the 8-tick decision path on one MI355X: arena-ticks/s over 65,536 contact-heavy arenas.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--block B]
+
+A step is one decision (8 ticks at 120 Hz, sim8) of all 65,536 arenas, with fresh per-car controls each decision, states
+resident in HBM (AoSoA-64). The timed region is K back-to-back decisions on the device stream, bracketed by device
+synchronisations and HIP events on that stream. Ticks after a goal are not stepped, so `value` counts the ticks actually
+stepped (the scored tick ends an arena's decision); the nominal 8-tick count is reported beside it.
+
+Workload: the contact-heavy mix (40% ball pinched against a wall or fillet with a car driving into it, 30% head-on cars,
+30% ball at the goal frame), Octane vs Octane on the procedural Soccar arena, seeded synthetic states and controls.
+
+Prints one JSON line. `roofline` prices the sim8 kernel against HBM at 20,544.5 algorithmic bytes per arena-tick (the
+2,567 persistent state words read and written once per tick, plus controls and result amortised over the decision).
+`cpu_baseline` is the CPU build of the same tick (the test oracle, g++ -O2, strict IEEE) on the host's cores over the
+first arenas of the same run; its outputs are word-compared with the GPU's (`word_diffs`).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rocketsim-cuda_amd"))
+
+ARENAS = 65536
+B_TICK = 2 * 2567 * 4 + (64 + 4) / 8  # algorithmic HBM bytes per arena-tick
+HBM_PEAK = 8.0e12
+METRIC = "arena-ticks/s @65,536 arenas, plain vs cost-sorted; 0 word diffs vs CPU"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sim8_contact65536_traffic.json")
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline_leg(world, states0, controls, gpu_states, gpu_results, sample, threads):
+    """The CPU build over arenas [0, sample) of the same run: timing plus a word compare with the GPU's outputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    if not os.path.exists(orc.ORACLE_LIB):
+        orc.build(ref=False)
+    o = orc.Oracle()
+    s = np.ascontiguousarray(states0[:sample])
+    ticks = 0
+    diffs = 0
+    t0 = time.perf_counter()
+    res = []
+    for c in controls:
+        r = o.sim8(s, world, np.ascontiguousarray(c[:sample]), threads=threads)
+        res.append(r)
+    dt = time.perf_counter() - t0
+    for k, r in enumerate(res):
+        sc = r & 0xFF
+        ticks += int(np.where(sc > 0, sc, 8).sum())
+        diffs += int((r != gpu_results[k][:sample]).sum())
+    diffs += int((s[:, :2567] != gpu_states[:sample, :2567]).sum())
+    return {"value": ticks / dt, "unit": "arena-ticks/s", "cores": threads, "kind": "port",
+            "sample": "arenas 0..%d of the same run over all %d timed decisions (%d arena-ticks), %d host threads, "
+                      "CPU build of the same tick (g++ -O2 -ffp-contract=off)" % (sample - 1, len(controls), ticks, threads),
+            "seconds": dt}, diffs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--arenas", type=int, default=ARENAS)
+    ap.add_argument("--cpu-sample", type=int, default=ARENAS)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this stepper runs on one GPU; --gpus %d is not supported (arenas are independent, so N GPUs would "
+              "each step their own range of AoSoA blocks)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import rsim
+
+    n, K, W = args.arenas, args.steps, args.warmup
+    print("[bench] building the world and %d contact-heavy arenas" % n, file=sys.stderr, flush=True)
+    world = rsim.build_world(0, 0)
+    states0 = rsim.sample_arenas(world, rsim.SCEN_CONTACT, 2025, n)
+    controls = [rsim.sample_controls(4242, d, n) for d in range(W + K)]
+
+    dev = rsim.SimDevice(0)
+    dw = dev.upload_static(world)
+    d_init = dev.upload_states(states0)
+    d_work = dev.alloc_states(n)
+    d_ctl = [dev.upload_controls(c) for c in controls]
+    d_res = [dev.alloc_words(n) for _ in range(K)]
+    d_warm = dev.alloc_words(n)
+
+    print("[bench] warmup %d decisions" % W, file=sys.stderr, flush=True)
+    dev.copy_states(d_work, d_init)
+    for k in range(W):
+        dev.sim8(d_work, dw, d_ctl[K + k], d_warm, args.block)
+    dev.sync()
+
+    print("[bench] timing %d decisions" % K, file=sys.stderr, flush=True)
+    dev.copy_states(d_work, d_init)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.timer_begin()
+    for k in range(K):
+        dev.sim8(d_work, dw, d_ctl[k], d_res[k], args.block)
+    gpu_ms = dev.timer_end()  # synchronises
+    wall = time.perf_counter() - t0
+
+    results = [dev.read_words(r) for r in d_res]
+    final = dev.download_states(d_work)
+    stepped = sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
+    nominal = n * 8 * K
+    flags = sum(int(((r & 0xC0000000) != 0).sum()) for r in results)
+    ms_step = wall * 1000.0 / K
+    value = stepped / wall
+    per_launch_s = gpu_ms / 1000.0 / K
+    achieved = B_TICK * (stepped / K) / per_launch_s
+    b_dec = (2 * 2567 * 4 + 68) / 8.0
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu, diffs = None, None
+    if not args.no_cpu:
+        print("[bench] CPU baseline on %d arenas" % min(args.cpu_sample, n), file=sys.stderr, flush=True)
+        cpu, diffs = cpu_baseline_leg(world, states0, controls[:K], final, results, min(args.cpu_sample, n), host_threads())
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "arena-ticks/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded contact-heavy start states and per-decision controls on the procedural Soccar arena",
+        "config": {"workload": "contact-heavy 65,536 arenas (40% wall/fillet pinch, 30% head-on, 30% goal frame), "
+                               "Octane vs Octane, 8-tick decisions, plain per-launch sim8",
+                   "arenas": n, "decisions": K, "block": args.block, "order": "plain (unsorted)"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "bytes_per_arena_tick": B_TICK, "kernel": "k_sim8", "kernel_ms_avg": gpu_ms / K,
+                     "decision_boundary_bytes_per_arena_tick": b_dec,
+                     "frac_at_decision_boundary_bytes": b_dec * (stepped / K) / per_launch_s / HBM_PEAK},
+        "cpu_baseline": cpu,
+        "arena_ticks_stepped": stepped,
+        "arena_ticks_nominal": nominal,
+        "nominal_arena_ticks_per_s": nominal / wall,
+        "flagged_arenas": flags,
+        "word_diffs": diffs,
+    }
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

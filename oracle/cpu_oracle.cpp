@@ -4,6 +4,9 @@
 // the same path (crates/rocketsim-cuda/csrc/phase.cpp:8-40: rscuda_phase, rscuda_step_tick, rscuda_step_ticks, lf_*). Tests
 // pin it bit for bit to the reference's own C++ build (oracle/_ref) on golden vectors, then check the gfx950 kernels against it.
 // Built by oracle/Makefile with g++ -O2 -ffp-contract=off (no FMA target).
+#include <thread>
+#include <vector>
+
 #include "core/rs_tick.h"
 
 using namespace rs;
@@ -28,6 +31,19 @@ int rsim_cpu_step_ticks(ArenaState* s, const StaticData* w, const Controls* cont
 void rsim_cpu_sim8_aos(ArenaState* states, const StaticData* w, const Controls* controls, uint32_t* result, uint32_t n) {
   for (uint32_t a = 0; a < n; ++a) result[a] = run_decision(states[a], *w, controls + 2 * (size_t)a, TICKS_PER_DECISION, NoTrace{});
 }
+// The same over `threads` host threads (arena ranges split evenly); the CPU baseline of the benchmark.
+void rsim_cpu_sim8_batch(ArenaState* states, const StaticData* w, const Controls* controls, uint32_t* result, uint32_t n,
+                         uint32_t threads) {
+  if (threads < 1) threads = 1;
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < threads; ++t) {
+    uint32_t lo = (uint32_t)((uint64_t)n * t / threads), hi = (uint32_t)((uint64_t)n * (t + 1) / threads);
+    pool.emplace_back([=] { rsim_cpu_sim8_aos(states + lo, w, controls + 2 * (size_t)lo, result + lo, hi - lo); });
+  }
+  for (auto& th : pool) th.join();
+}
+// Word offset of cars[car].controls in ArenaState (for setting controls before a per-phase run).
+uint64_t rsim_cpu_controls_word(int car) { return (offsetof(ArenaState, cars) + car * sizeof(Car) + offsetof(Car, controls)) / 4; }
 uint64_t rsim_cpu_arena_words() { return ARENA_WORDS; }
 uint64_t rsim_cpu_tick_words() { return TICK_WORDS; }
 uint64_t rsim_cpu_static_words() { return STATIC_WORDS; }

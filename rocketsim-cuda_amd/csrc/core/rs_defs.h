@@ -15,7 +15,7 @@
 #include <hip/hip_runtime.h>
 #define RS_HD __host__ __device__
 #define RS_FN __host__ __device__ inline
-#define RS_NOINL __host__ __device__ __attribute__((noinline))
+#define RS_NOINL __host__ __device__ __attribute__((noinline)) inline
 #define RS_TABLE static __device__ __constant__ const
 #else
 #define RS_HD

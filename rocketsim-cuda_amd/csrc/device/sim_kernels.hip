@@ -1,0 +1,131 @@
+// The gfx950 kernels of the decision path: one lane per arena over an AoSoA-64 state buffer.
+//
+// Word w of arena a sits at (a / 64) * ARENA_WORDS * 64 + w * 64 + a % 64, so the 64 lanes of a wave that copy their arenas'
+// word w touch one contiguous 256-byte run: every copy-in and copy-out instruction is one fully coalesced HBM transaction.
+// Each lane steps its arena in a private ArenaState (the physics core of rs_tick.h, the same source the CPU oracle compiles);
+// no lane reads another's arena, so neither the work-group size nor the arena order can change a result.
+//
+// Kernels (reference cuda/sim/sim.cu:8-112):
+//   rs_identity / rs_identity_tick   copy-in and copy-out only (the layout check)
+//   rs_phase                          one phase k of the tick on whole states (per-phase parity)
+//   rs_step_tick                      one full tick at a tick boundary
+//   rs_sim8 / rs_sim8_trace           one decision: 8 ticks, stopping after the scoring tick
+#include "device/launch.h"
+
+namespace rs {
+namespace dev {
+
+RS_HD inline size_t block_base(uint32_t a) { return (size_t)(a / AOSOA_LANES) * ARENA_WORDS * AOSOA_LANES + (a % AOSOA_LANES); }
+
+__device__ inline void load_all(ArenaState& s, const uint32_t* __restrict__ buf, uint32_t a) {
+  uint32_t* d = reinterpret_cast<uint32_t*>(&s);
+  const uint32_t* src = buf + block_base(a);
+  for (int w = 0; w < ARENA_WORDS; ++w) d[w] = src[(size_t)w * AOSOA_LANES];
+}
+__device__ inline void store_all(uint32_t* __restrict__ buf, uint32_t a, const ArenaState& s) {
+  const uint32_t* d = reinterpret_cast<const uint32_t*>(&s);
+  uint32_t* dst = buf + block_base(a);
+  for (int w = 0; w < ARENA_WORDS; ++w) dst[(size_t)w * AOSOA_LANES] = d[w];
+}
+// Tick-boundary copies: the TICK_WORDS persistent words move; the mid-tick tail is zero between ticks (arena_state.cuh:221).
+__device__ inline void load_tick(ArenaState& s, const uint32_t* __restrict__ buf, uint32_t a) {
+  uint32_t* d = reinterpret_cast<uint32_t*>(&s);
+  const uint32_t* src = buf + block_base(a);
+  for (int w = 0; w < TICK_WORDS; ++w) d[w] = src[(size_t)w * AOSOA_LANES];
+  for (int w = TICK_WORDS; w < ARENA_WORDS; ++w) d[w] = 0;
+}
+__device__ inline void store_tick(uint32_t* __restrict__ buf, uint32_t a, const ArenaState& s) {
+  const uint32_t* d = reinterpret_cast<const uint32_t*>(&s);
+  uint32_t* dst = buf + block_base(a);
+  for (int w = 0; w < TICK_WORDS; ++w) dst[(size_t)w * AOSOA_LANES] = d[w];
+}
+
+__global__ void __launch_bounds__(MAX_BLOCK) k_identity(const uint32_t* in, uint32_t* out, uint32_t n) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_all(s, in, a);
+  store_all(out, a, s);
+}
+__global__ void __launch_bounds__(MAX_BLOCK) k_identity_tick(const uint32_t* in, uint32_t* out, uint32_t n) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_tick(s, in, a);
+  store_tick(out, a, s);
+}
+__global__ void __launch_bounds__(MAX_BLOCK) k_phase(int k, uint32_t* buf, const StaticData* __restrict__ w, uint32_t n,
+                                                     int32_t* status) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_all(s, buf, a);
+  status[a] = run_phase(k, s, *w);
+  store_all(buf, a, s);
+}
+__global__ void __launch_bounds__(MAX_BLOCK) k_step_tick(uint32_t* buf, const StaticData* __restrict__ w, uint32_t n,
+                                                         int32_t* status) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_tick(s, buf, a);
+  status[a] = step_tick(s, *w);
+  store_tick(buf, a, s);
+}
+
+struct TraceOut {  // the tick-boundary words after tick t into slab t - 1 of the trace
+  uint32_t* trace;
+  size_t slab_words;
+  uint32_t a;
+  __device__ void operator()(uint32_t t, const ArenaState& s) const { store_tick(trace + (size_t)(t - 1) * slab_words, a, s); }
+};
+
+__global__ void __launch_bounds__(MAX_BLOCK) k_sim8(uint32_t* buf, const StaticData* __restrict__ w,
+                                                    const Controls* __restrict__ ctl, uint32_t* result, uint32_t n) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_tick(s, buf, a);
+  Controls c[NUM_CARS];
+  for (int i = 0; i < NUM_CARS; ++i) c[i] = ctl[(size_t)a * NUM_CARS + i];
+  result[a] = run_decision(s, *w, c, TICKS_PER_DECISION, NoTrace{});
+  store_tick(buf, a, s);
+}
+__global__ void __launch_bounds__(MAX_BLOCK) k_sim8_trace(uint32_t* buf, const StaticData* __restrict__ w,
+                                                          const Controls* __restrict__ ctl, uint32_t* result, uint32_t n,
+                                                          uint32_t* trace) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ArenaState s;
+  load_tick(s, buf, a);
+  Controls c[NUM_CARS];
+  for (int i = 0; i < NUM_CARS; ++i) c[i] = ctl[(size_t)a * NUM_CARS + i];
+  result[a] = run_decision(s, *w, c, TICKS_PER_DECISION, TraceOut{trace, aosoa_words(n), a});
+  store_tick(buf, a, s);
+}
+
+// ---- launchers ------------------------------------------------------------------------------------------------------------------
+static dim3 grid_of(uint32_t n, uint32_t block) { return dim3((n + block - 1) / block > 0 ? (n + block - 1) / block : 1); }
+
+hipError_t launch_identity(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, bool tick_only) {
+  if (tick_only) k_identity_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(in, out, n);
+  else k_identity<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_phase(hipStream_t st, int k, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status) {
+  k_phase<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(k, buf, w, n, status);
+  return hipGetLastError();
+}
+hipError_t launch_step_tick(hipStream_t st, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status) {
+  k_step_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(buf, w, n, status);
+  return hipGetLastError();
+}
+hipError_t launch_sim8(hipStream_t st, uint32_t* buf, const StaticData* w, const Controls* ctl, uint32_t* result, uint32_t n,
+                       uint32_t block, uint32_t* trace) {
+  if (trace) k_sim8_trace<<<grid_of(n, block), block, 0, st>>>(buf, w, ctl, result, n, trace);
+  else k_sim8<<<grid_of(n, block), block, 0, st>>>(buf, w, ctl, result, n);
+  return hipGetLastError();
+}
+
+}  // namespace dev
+}  // namespace rs

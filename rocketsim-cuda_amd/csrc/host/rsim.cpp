@@ -1,0 +1,423 @@
+// The C ABI of librsim.so (include/rsim.h): device, buffers, uploads with the AoSoA-64 pack, the decision launch and the
+// parity kernels. Mirrors the reference's SimDevice (crates/rocketsim-cuda/src/device.rs:30-172): one stream per device,
+// every call ordered on it, host data returned only after a synchronisation. Misuse returns an error code instead of the
+// reference's assert! panics; nothing here throws across the ABI.
+#include "rsim.h"
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "device/launch.h"
+
+using rs::ArenaState;
+using rs::Controls;
+using rs::StaticData;
+
+struct rsim_device {
+  int ordinal;
+  hipStream_t stream;
+  hipEvent_t t0, t1;
+  int32_t* status;  // per-arena status words of the parity kernels
+  uint32_t status_cap;
+};
+struct rsim_world {
+  rsim_device* dev;
+  StaticData* d;
+};
+struct rsim_states {
+  rsim_device* dev;
+  uint32_t* d;
+  uint32_t n;
+};
+struct rsim_words {
+  rsim_device* dev;
+  uint32_t* d;
+  uint64_t n;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* what) {
+  g_error = what;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_error = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+  return RSIM_ERR_HIP;
+}
+#define RSIM_HIP(call, what)                         \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+// Makes dev's GPU current for this thread (HIP's current device is per thread).
+int use(rsim_device* dev) {
+  if (!dev) return fail(RSIM_ERR_INVALID, "null device");
+  RSIM_HIP(hipSetDevice(dev->ordinal), "hipSetDevice");
+  return RSIM_OK;
+}
+
+void pack(const ArenaState* states, uint32_t n, uint32_t* out) {
+  const size_t words = rs::dev::aosoa_words(n);
+  const uint32_t L = rs::dev::AOSOA_LANES;
+  for (uint32_t b = 0; b * L < n || (n == 0 && false); ++b) {
+    uint32_t* blk = out + (size_t)b * rs::ARENA_WORDS * L;
+    for (uint32_t l = 0; l < L; ++l) {
+      uint32_t a = b * L + l;
+      if (a < n) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&states[a]);
+        for (int w = 0; w < rs::ARENA_WORDS; ++w) blk[(size_t)w * L + l] = src[w];
+      } else {
+        for (int w = 0; w < rs::ARENA_WORDS; ++w) blk[(size_t)w * L + l] = 0;
+      }
+    }
+  }
+  (void)words;
+}
+void unpack(const uint32_t* in, uint32_t n, ArenaState* states) {
+  const uint32_t L = rs::dev::AOSOA_LANES;
+  for (uint32_t a = 0; a < n; ++a) {
+    const uint32_t* blk = in + (size_t)(a / L) * rs::ARENA_WORDS * L + (a % L);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&states[a]);
+    for (int w = 0; w < rs::ARENA_WORDS; ++w) dst[w] = blk[(size_t)w * L];
+  }
+}
+
+int ensure_status(rsim_device* dev, uint32_t n) {
+  if (dev->status_cap >= n && dev->status) return RSIM_OK;
+  if (dev->status) (void)hipFree(dev->status);
+  dev->status = nullptr;
+  dev->status_cap = 0;
+  uint32_t cap = n < 64 ? 64 : n;
+  RSIM_HIP(hipMalloc((void**)&dev->status, (size_t)cap * sizeof(int32_t)), "hipMalloc(status)");
+  dev->status_cap = cap;
+  return RSIM_OK;
+}
+// The status words of the last parity launch back to the host (synchronous).
+int finish_status(rsim_device* dev, uint32_t n, int32_t* status_out) {
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "parity kernel");
+  if (status_out && n)
+    RSIM_HIP(hipMemcpy(status_out, dev->status, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost), "status download");
+  return RSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- layout ------------------------------------------------------------------------------------------------------------------
+uint64_t rsim_arena_words(void) { return rs::ARENA_WORDS; }
+uint64_t rsim_tick_words(void) { return rs::TICK_WORDS; }
+uint64_t rsim_static_words(void) { return rs::STATIC_WORDS; }
+uint64_t rsim_aosoa_words(uint32_t n) { return rs::dev::aosoa_words(n); }
+uint64_t rsim_aosoa_index(uint32_t a, uint32_t w) { return rs::dev::aosoa_index(a, w); }
+int rsim_aosoa_pack(const void* states, uint32_t n, uint32_t* aosoa) {
+  if ((!states || !aosoa) && n) return fail(RSIM_ERR_INVALID, "rsim_aosoa_pack: null buffer");
+  pack(static_cast<const ArenaState*>(states), n, aosoa);
+  return RSIM_OK;
+}
+int rsim_aosoa_unpack(const uint32_t* aosoa, uint32_t n, void* states) {
+  if ((!states || !aosoa) && n) return fail(RSIM_ERR_INVALID, "rsim_aosoa_unpack: null buffer");
+  unpack(aosoa, n, static_cast<ArenaState*>(states));
+  return RSIM_OK;
+}
+const char* rsim_last_error(void) { return g_error.c_str(); }
+
+// ---- device ------------------------------------------------------------------------------------------------------------------
+int rsim_open(int ordinal, rsim_device** out) {
+  if (!out) return fail(RSIM_ERR_INVALID, "rsim_open: null out");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(RSIM_ERR_NO_DEVICE, "rsim_open: no GPU");
+  if (ordinal < 0 || ordinal >= count) return fail(RSIM_ERR_NO_DEVICE, "rsim_open: no such GPU");
+  RSIM_HIP(hipSetDevice(ordinal), "hipSetDevice");
+  rsim_device* d = new (std::nothrow) rsim_device{};
+  if (!d) return fail(RSIM_ERR_INVALID, "rsim_open: out of host memory");
+  d->ordinal = ordinal;
+  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&d->t0);
+  if (e == hipSuccess) e = hipEventCreate(&d->t1);
+  if (e != hipSuccess) {
+    delete d;
+    return hip_fail(e, "rsim_open");
+  }
+  *out = d;
+  return RSIM_OK;
+}
+int rsim_close(rsim_device* dev) {
+  if (!dev) return RSIM_OK;
+  int rc = use(dev);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(dev->stream);
+  if (dev->status) (void)hipFree(dev->status);
+  (void)hipEventDestroy(dev->t0);
+  (void)hipEventDestroy(dev->t1);
+  (void)hipStreamDestroy(dev->stream);
+  delete dev;
+  return RSIM_OK;
+}
+void* rsim_stream(rsim_device* dev) { return dev ? (void*)dev->stream : nullptr; }
+int rsim_sync(rsim_device* dev) {
+  int rc = use(dev);
+  if (rc) return rc;
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "rsim_sync");
+  return RSIM_OK;
+}
+int rsim_timer_begin(rsim_device* dev) {
+  int rc = use(dev);
+  if (rc) return rc;
+  RSIM_HIP(hipEventRecord(dev->t0, dev->stream), "hipEventRecord");
+  return RSIM_OK;
+}
+int rsim_timer_end(rsim_device* dev, float* ms) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!ms) return fail(RSIM_ERR_INVALID, "rsim_timer_end: null ms");
+  RSIM_HIP(hipEventRecord(dev->t1, dev->stream), "hipEventRecord");
+  RSIM_HIP(hipEventSynchronize(dev->t1), "hipEventSynchronize");
+  RSIM_HIP(hipEventElapsedTime(ms, dev->t0, dev->t1), "hipEventElapsedTime");
+  return RSIM_OK;
+}
+
+// ---- world -------------------------------------------------------------------------------------------------------------------
+int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!static_data || !out) return fail(RSIM_ERR_INVALID, "rsim_upload_world: null argument");
+  rsim_world* w = new (std::nothrow) rsim_world{dev, nullptr};
+  if (!w) return fail(RSIM_ERR_INVALID, "out of host memory");
+  hipError_t e = hipMalloc((void**)&w->d, sizeof(StaticData));
+  if (e == hipSuccess) e = hipMemcpyAsync(w->d, static_data, sizeof(StaticData), hipMemcpyHostToDevice, dev->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+  if (e != hipSuccess) {
+    if (w->d) (void)hipFree(w->d);
+    delete w;
+    return hip_fail(e, "rsim_upload_world");
+  }
+  *out = w;
+  return RSIM_OK;
+}
+int rsim_world_free(rsim_world* w) {
+  if (!w) return RSIM_OK;
+  int rc = use(w->dev);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(w->dev->stream);
+  (void)hipFree(w->d);
+  delete w;
+  return RSIM_OK;
+}
+
+// ---- states ------------------------------------------------------------------------------------------------------------------
+int rsim_alloc_states(rsim_device* dev, uint32_t n, rsim_states** out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!out) return fail(RSIM_ERR_INVALID, "rsim_alloc_states: null out");
+  size_t bytes = rs::dev::aosoa_words(n > 0 ? n : 1) * sizeof(uint32_t);
+  rsim_states* s = new (std::nothrow) rsim_states{dev, nullptr, n};
+  if (!s) return fail(RSIM_ERR_INVALID, "out of host memory");
+  hipError_t e = hipMalloc((void**)&s->d, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(s->d, 0, bytes, dev->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+  if (e != hipSuccess) {
+    if (s->d) (void)hipFree(s->d);
+    delete s;
+    return hip_fail(e, "rsim_alloc_states");
+  }
+  *out = s;
+  return RSIM_OK;
+}
+int rsim_write_states(rsim_device* dev, rsim_states* dst, const void* states) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!dst || (!states && dst->n)) return fail(RSIM_ERR_INVALID, "rsim_write_states: null argument");
+  std::vector<uint32_t> host;
+  try {
+    host.resize(rs::dev::aosoa_words(dst->n));
+  } catch (...) {
+    return fail(RSIM_ERR_INVALID, "rsim_write_states: out of host memory");
+  }
+  pack(static_cast<const ArenaState*>(states), dst->n, host.data());
+  RSIM_HIP(hipMemcpyAsync(dst->d, host.data(), host.size() * 4, hipMemcpyHostToDevice, dev->stream), "state upload");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "state upload");
+  return RSIM_OK;
+}
+int rsim_upload_states(rsim_device* dev, const void* states, uint32_t n, rsim_states** out) {
+  int rc = rsim_alloc_states(dev, n, out);
+  if (rc) return rc;
+  rc = rsim_write_states(dev, *out, states);
+  if (rc) {
+    rsim_states_free(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+int rsim_copy_states(rsim_device* dev, rsim_states* dst, const rsim_states* src) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!dst || !src || dst->n != src->n) return fail(RSIM_ERR_INVALID, "rsim_copy_states: buffers of different sizes");
+  RSIM_HIP(hipMemcpyAsync(dst->d, src->d, rs::dev::aosoa_words(src->n) * 4, hipMemcpyDeviceToDevice, dev->stream),
+           "state copy");
+  return RSIM_OK;
+}
+int rsim_states_free(rsim_states* s) {
+  if (!s) return RSIM_OK;
+  int rc = use(s->dev);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(s->dev->stream);
+  (void)hipFree(s->d);
+  delete s;
+  return RSIM_OK;
+}
+uint32_t rsim_states_count(const rsim_states* s) { return s ? s->n : 0; }
+void* rsim_states_device_ptr(rsim_states* s) { return s ? s->d : nullptr; }
+int rsim_download_words(rsim_device* dev, const rsim_states* s, uint32_t* out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!s || (!out && s->n)) return fail(RSIM_ERR_INVALID, "rsim_download_words: null argument");
+  if (!s->n) return RSIM_OK;
+  RSIM_HIP(hipMemcpyAsync(out, s->d, rs::dev::aosoa_words(s->n) * 4, hipMemcpyDeviceToHost, dev->stream), "state download");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "state download");
+  return RSIM_OK;
+}
+int rsim_download_states(rsim_device* dev, const rsim_states* s, void* out) {
+  if (!s) return fail(RSIM_ERR_INVALID, "rsim_download_states: null states");
+  std::vector<uint32_t> host;
+  try {
+    host.resize(rs::dev::aosoa_words(s->n));
+  } catch (...) {
+    return fail(RSIM_ERR_INVALID, "rsim_download_states: out of host memory");
+  }
+  int rc = rsim_download_words(dev, s, host.data());
+  if (rc) return rc;
+  unpack(host.data(), s->n, static_cast<ArenaState*>(out));
+  return RSIM_OK;
+}
+
+// ---- word buffers ------------------------------------------------------------------------------------------------------------
+int rsim_alloc_words(rsim_device* dev, uint64_t words, rsim_words** out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!out) return fail(RSIM_ERR_INVALID, "rsim_alloc_words: null out");
+  rsim_words* b = new (std::nothrow) rsim_words{dev, nullptr, words};
+  if (!b) return fail(RSIM_ERR_INVALID, "out of host memory");
+  size_t bytes = (words > 0 ? words : 1) * 4;
+  hipError_t e = hipMalloc((void**)&b->d, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(b->d, 0, bytes, dev->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+  if (e != hipSuccess) {
+    if (b->d) (void)hipFree(b->d);
+    delete b;
+    return hip_fail(e, "rsim_alloc_words");
+  }
+  *out = b;
+  return RSIM_OK;
+}
+int rsim_write_words(rsim_device* dev, rsim_words* dst, const uint32_t* host, uint64_t words) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!dst || (!host && words)) return fail(RSIM_ERR_INVALID, "rsim_write_words: null argument");
+  if (words > dst->n) return fail(RSIM_ERR_SHORT, "rsim_write_words: buffer too short");
+  if (!words) return RSIM_OK;
+  RSIM_HIP(hipMemcpyAsync(dst->d, host, words * 4, hipMemcpyHostToDevice, dev->stream), "word upload");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "word upload");
+  return RSIM_OK;
+}
+int rsim_upload_words(rsim_device* dev, const uint32_t* host, uint64_t words, rsim_words** out) {
+  int rc = rsim_alloc_words(dev, words, out);
+  if (rc) return rc;
+  rc = rsim_write_words(dev, *out, host, words);
+  if (rc) {
+    rsim_words_free(*out);
+    *out = nullptr;
+  }
+  return rc;
+}
+int rsim_read_words(rsim_device* dev, const rsim_words* src, uint32_t* host, uint64_t words) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!src || (!host && words)) return fail(RSIM_ERR_INVALID, "rsim_read_words: null argument");
+  if (words > src->n) return fail(RSIM_ERR_SHORT, "rsim_read_words: buffer too short");
+  if (!words) return RSIM_OK;
+  RSIM_HIP(hipMemcpyAsync(host, src->d, words * 4, hipMemcpyDeviceToHost, dev->stream), "word download");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "word download");
+  return RSIM_OK;
+}
+int rsim_upload_controls(rsim_device* dev, const void* controls, uint32_t n, rsim_words** out) {
+  static_assert(sizeof(Controls) == 32, "Controls is eight words");
+  return rsim_upload_words(dev, static_cast<const uint32_t*>(controls), (uint64_t)n * rs::NUM_CARS * 8, out);
+}
+int rsim_words_free(rsim_words* b) {
+  if (!b) return RSIM_OK;
+  int rc = use(b->dev);
+  if (rc) return rc;
+  (void)hipStreamSynchronize(b->dev->stream);
+  (void)hipFree(b->d);
+  delete b;
+  return RSIM_OK;
+}
+uint64_t rsim_words_count(const rsim_words* b) { return b ? b->n : 0; }
+void* rsim_words_device_ptr(rsim_words* b) { return b ? b->d : nullptr; }
+
+// ---- kernels -----------------------------------------------------------------------------------------------------------------
+int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
+              uint32_t block, rsim_words* trace) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world || !controls || !result) return fail(RSIM_ERR_INVALID, "rsim_sim8: null argument");
+  if (block < 1 || block > RSIM_MAX_BLOCK) return fail(RSIM_ERR_INVALID, "rsim_sim8: block outside 1..RSIM_MAX_BLOCK");
+  const uint32_t n = states->n;
+  if (controls->n < (uint64_t)n * rs::NUM_CARS * 8 || result->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8: buffers too short");
+  if (trace && trace->n < (uint64_t)RSIM_TICKS_PER_DECISION * rs::dev::aosoa_words(n))
+    return fail(RSIM_ERR_SHORT, "rsim_sim8: trace too short");
+  if (!n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_sim8(dev->stream, states->d, world->d, reinterpret_cast<const Controls*>(controls->d), result->d, n,
+                                block, trace ? trace->d : nullptr),
+           "sim8 launch");
+  return RSIM_OK;
+}
+int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!in || !out || in->n != out->n) return fail(RSIM_ERR_INVALID, "rsim_identity: buffers of different sizes");
+  if (!in->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_identity(dev->stream, in->d, out->d, in->n, false), "identity launch");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "identity");
+  return RSIM_OK;
+}
+int rsim_identity_tick(rsim_device* dev, const rsim_states* in, rsim_states* out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!in || !out || in->n != out->n) return fail(RSIM_ERR_INVALID, "rsim_identity_tick: buffers of different sizes");
+  if (!in->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_identity(dev->stream, in->d, out->d, in->n, true), "identity_tick launch");
+  RSIM_HIP(hipStreamSynchronize(dev->stream), "identity_tick");
+  return RSIM_OK;
+}
+int rsim_phase(rsim_device* dev, int k, rsim_states* states, const rsim_world* world, int32_t* status_out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world) return fail(RSIM_ERR_INVALID, "rsim_phase: null argument");
+  if (!states->n) return RSIM_OK;
+  if ((rc = ensure_status(dev, states->n))) return rc;
+  RSIM_HIP(rs::dev::launch_phase(dev->stream, k, states->d, world->d, states->n, dev->status), "phase launch");
+  return finish_status(dev, states->n, status_out);
+}
+int rsim_step_tick(rsim_device* dev, rsim_states* states, const rsim_world* world, int32_t* status_out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world) return fail(RSIM_ERR_INVALID, "rsim_step_tick: null argument");
+  if (!states->n) return RSIM_OK;
+  if ((rc = ensure_status(dev, states->n))) return rc;
+  RSIM_HIP(rs::dev::launch_step_tick(dev->stream, states->d, world->d, states->n, dev->status), "step_tick launch");
+  return finish_status(dev, states->n, status_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+"""Builds librsim.so in-tree: the gfx950 kernels and the C ABI (hipcc) plus the host-side world builder (g++).
+
+    python -m rsim.build            (from rocketsim-cuda_amd/), or rsim.build.build() from Python
+
+Float policy of both compilers: -ffp-contract=off (no fused multiply-add), no fast-math, fp32 denormals kept, HIP's correctly
+rounded float divide and sqrt. All HIP code is compiled without packed-fp32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "librsim.so")
+
+ARCH = "gfx950"
+COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + CSRC, "-I" + INCLUDE]
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH,
+    "-fno-gpu-flush-denormals-to-zero",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+HIP_SOURCES = [os.path.join(CSRC, "device", "sim_kernels.hip"), os.path.join(CSRC, "host", "rsim.cpp")]
+HOST_SOURCES = [os.path.join(CSRC, "host", "world.cpp")]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found: librsim.so needs the ROCm toolchain to build")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed (%d):\n%s\n%s" % (r.returncode, " ".join(cmd), r.stdout[-20000:]))
+    return r.stdout
+
+
+def build(verbose=False):
+    """Compiles every source and links librsim.so (always rebuilds)."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([hipcc] + COMMON + HIP_FLAGS + lang + ["-c", src, "-o", obj])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+        _run(["g++"] + COMMON + ["-c", src, "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,230 @@
+"""Parity of the gfx950 kernels with the CPU oracle, through librsim.so's C ABI: every TICK_WORDS word of every arena and
+every result word must be identical (the oracle is itself pinned to the reference's C++ build by the golden vectors)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import first_diff
+from helpers import TICK_WORDS, digest
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_decisions(rsim, device, dev_world, states, controls, block=64, trace=False):
+    n = states.shape[0]
+    d = device.upload_states(states)
+    r = device.alloc_words(n)
+    results, traces = [], []
+    for ctl in controls:
+        c = device.upload_controls(ctl)
+        t = device.alloc_words(8 * rsim.aosoa_words(n)) if trace else None
+        device.sim8(d, dev_world, c, r, block, t)
+        device.sync()
+        results.append(device.read_words(r))
+        if trace:
+            w = device.read_words(t)
+            slabs = w.reshape(8, -1)
+            traces.append(np.stack([rsim.aosoa_unpack(slabs[k], n) for k in range(8)], 1))
+    return device.download_states(d), results, traces
+
+
+def cpu_decisions(oracle, world, states, controls):
+    s = states.copy()
+    res = [oracle.sim8(s, world, ctl, threads=8) for ctl in controls]
+    return s, res
+
+
+def assert_same(gs, gr, cs, cr, what):
+    for k, (a, b) in enumerate(zip(gr, cr)):
+        if not np.array_equal(a, b):
+            i = int(np.argmax(a != b))
+            pytest.fail("%s: decision %d result word of arena %d: gpu %#x cpu %#x" % (what, k, i, a[i], b[i]))
+    fd = first_diff(gs[:, :TICK_WORDS], cs[:, :TICK_WORDS])
+    assert fd is None, "%s: arena %d word %d differs (gpu %#x cpu %#x)" % (what, fd[0], fd[1], gs[fd], cs[fd])
+    assert not gs[:, TICK_WORDS:].any(), "%s: the mid-tick tail is not zero on the device" % what
+
+
+def test_identity_round_trip(rsim, device):
+    note("identity round trip, 300 random states")
+    rng = np.random.default_rng(1)
+    states = rng.integers(0, 2**32, (300, rsim.ARENA_WORDS), dtype=np.uint64).astype(np.uint32)
+    a = device.upload_states(states)
+    b = device.alloc_states(300)
+    device.identity(a, b)
+    assert np.array_equal(device.download_states(b), states)
+    device.identity_tick(a, b)
+    got = device.download_states(b)
+    assert np.array_equal(got[:, :TICK_WORDS], states[:, :TICK_WORDS])
+    # AoSoA placement: word w of arena i at (i / 64) * 2784 * 64 + w * 64 + i % 64
+    words = device.download_words(a)
+    for i, w in [(0, 0), (63, 5), (64, 0), (299, 2783), (130, 1000)]:
+        assert words[rsim.aosoa_index(i, w)] == states[i, w]
+        assert rsim.aosoa_index(i, w) == (i // 64) * 2784 * 64 + w * 64 + i % 64
+
+
+def test_kickoff_1024(rsim, oracle, world, device, dev_world):
+    note("kickoff 1024 arenas x 4 decisions")
+    states = rsim.sample_arenas(world, rsim.SCEN_KICKOFF, 11, 1024)
+    ctl = [rsim.sample_controls(12, d, 1024) for d in range(4)]
+    gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+    cs, cr = cpu_decisions(oracle, world, states, ctl)
+    assert_same(gs, gr, cs, cr, "kickoff")
+
+
+@pytest.mark.parametrize("scenario", [1, 2, 3, 4, 5])
+def test_scenarios(rsim, oracle, world, device, dev_world, scenario):
+    note("scenario %d, 2048 arenas x 3 decisions" % scenario)
+    states = rsim.sample_arenas(world, scenario, 100 + scenario, 2048)
+    ctl = [rsim.sample_controls(200 + scenario, d, 2048) for d in range(3)]
+    gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+    cs, cr = cpu_decisions(oracle, world, states, ctl)
+    assert_same(gs, gr, cs, cr, "scenario %d" % scenario)
+
+
+@pytest.mark.parametrize("presets", [(1, 2), (3, 4), (5, 6), (6, 0)])
+def test_presets(rsim, oracle, device, presets):
+    note("presets %s, contact mix, 512 arenas x 3 decisions" % (presets,))
+    w = rsim.build_world(*presets)
+    dw = device.upload_static(w)
+    states = rsim.sample_arenas(w, rsim.SCEN_MIXED, 300 + presets[0], 512)
+    ctl = [rsim.sample_controls(301, d, 512) for d in range(3)]
+    gs, gr, _ = gpu_decisions(rsim, device, dw, states, ctl)
+    cs, cr = cpu_decisions(oracle, w, states, ctl)
+    assert_same(gs, gr, cs, cr, "presets %s" % (presets,))
+
+
+def test_mutators(rsim, oracle, device):
+    note("mutator worlds (gravity, boost, ball radius)")
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        g, b, r = float(rng.uniform(-800, -500)), float(rng.uniform(0.8, 1.2)), float(rng.uniform(91.25, 100))
+        w = rsim.build_world(k, 6 - k, g, b, r)
+        dw = device.upload_static(w)
+        states = rsim.sample_arenas(w, rsim.SCEN_MIXED, 400 + k, 256)
+        ctl = [rsim.sample_controls(401, d, 256) for d in range(2)]
+        gs, gr, _ = gpu_decisions(rsim, device, dw, states, ctl)
+        cs, cr = cpu_decisions(oracle, w, states, ctl)
+        assert_same(gs, gr, cs, cr, "mutators %d" % k)
+
+
+def test_trace(rsim, oracle, world, device, dev_world):
+    note("sim8 trace, 256 arenas of the contact mix")
+    n = 256
+    states = rsim.sample_arenas(world, rsim.SCEN_CONTACT, 21, n)
+    ctl = rsim.sample_controls(22, 0, n)
+    gs, gr, gt = gpu_decisions(rsim, device, dev_world, states, [ctl], trace=True)
+    for a in range(n):
+        s = states[a].copy()
+        tr = np.zeros((8, rsim.ARENA_WORDS), np.uint32)
+        st, scored = oracle.step_ticks(s, world, ctl[a], 8, tr)
+        last = scored if scored else 8
+        for t in range(last):
+            assert np.array_equal(gt[0][a, t, :TICK_WORDS], tr[t, :TICK_WORDS]), "arena %d tick %d" % (a, t + 1)
+        for t in range(last, 8):  # ticks after a goal are not written
+            assert not gt[0][a, t].any(), "arena %d: tick %d after the goal was written" % (a, t + 1)
+
+
+def test_step_tick_and_phases(rsim, oracle, world, device, dev_world):
+    note("step_tick and the ten phases, 256 arenas")
+    n = 256
+    states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 31, n)
+    ctl = rsim.sample_controls(32, 0, n)
+    for a in range(n):
+        oracle.set_controls(states[a], ctl[a])
+    # whole tick
+    d = device.upload_states(states)
+    status = device.step_tick(d, dev_world)
+    got = device.download_states(d)
+    for a in range(n):
+        s = states[a].copy()
+        assert oracle.step_tick(s, world) == status[a]
+        assert np.array_equal(got[a, :TICK_WORDS], s[:TICK_WORDS]), "step_tick arena %d" % a
+    # phase by phase on whole states (the mid-tick tail moves between phases)
+    d = device.upload_states(states)
+    cpu = states.copy()
+    for k in range(10):
+        st = device.phase(k, d, dev_world)
+        got = device.download_states(d)
+        for a in range(n):
+            assert oracle.phase(k, cpu[a], world) == st[a]
+        fd = first_diff(got, cpu)
+        assert fd is None, "phase %d: arena %d word %d" % (k, fd[0], fd[1])
+    assert (device.phase(10, d, dev_world) == -2).all()  # PHASE_BAD_INDEX
+
+
+@pytest.mark.parametrize("block", [1, 7, 32, 64, 128, 256])
+def test_block_size_invariance(rsim, world, device, dev_world, block):
+    note("block size %d" % block)
+    n = 1000
+    states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 41, n)
+    ctl = [rsim.sample_controls(42, d, n) for d in range(2)]
+    base = gpu_decisions(rsim, device, dev_world, states, ctl, block=64)
+    got = gpu_decisions(rsim, device, dev_world, states, ctl, block=block)
+    assert np.array_equal(base[0][:, :TICK_WORDS], got[0][:, :TICK_WORDS])
+    for a, b in zip(base[1], got[1]):
+        assert np.array_equal(a, b)
+
+
+def test_permutation_invariance(rsim, world, device, dev_world):
+    note("permutation invariance, 1000 arenas")
+    n = 1000
+    states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 51, n)
+    ctl = [rsim.sample_controls(52, d, n) for d in range(2)]
+    perm = np.random.default_rng(53).permutation(n)
+    base = gpu_decisions(rsim, device, dev_world, states, ctl)
+    got = gpu_decisions(rsim, device, dev_world, states[perm].copy(), [c[perm].copy() for c in ctl])
+    assert np.array_equal(base[0][perm, :TICK_WORDS], got[0][:, :TICK_WORDS])
+    for a, b in zip(base[1], got[1]):
+        assert np.array_equal(a[perm], b)
+
+
+def test_contact_heavy_full_size(rsim, oracle, world, device, dev_world):
+    note("contact-heavy 65,536 arenas x 1 decision, every word")
+    n = 65536
+    states = rsim.sample_arenas(world, rsim.SCEN_CONTACT, 61, n)
+    ctl = [rsim.sample_controls(62, 0, n)]
+    gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+    cs, cr = cpu_decisions(oracle, world, states, ctl)
+    assert_same(gs, gr, cs, cr, "contact 65536")
+    # a size-independent check: the digest of digests is stable across block sizes
+    gs2, gr2, _ = gpu_decisions(rsim, device, dev_world, states, ctl, block=128)
+    assert digest(gs2[:, :TICK_WORDS]) == digest(gs[:, :TICK_WORDS]) and np.array_equal(gr2[0], gr[0])
+
+
+def test_edge_sizes(rsim, oracle, world, device, dev_world):
+    note("ragged sizes 1, 63, 65 and an empty batch")
+    for n in (1, 63, 65):
+        states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 70 + n, n)
+        ctl = [rsim.sample_controls(71, 0, n)]
+        gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+        cs, cr = cpu_decisions(oracle, world, states, ctl)
+        assert_same(gs, gr, cs, cr, "n=%d" % n)
+    d = device.alloc_states(0)
+    c = device.alloc_words(16)
+    r = device.alloc_words(1)
+    device.sim8(d, dev_world, c, r)  # no arenas: nothing to launch
+    device.sync()
+
+
+def test_argument_checks(rsim, world, device, dev_world):
+    note("argument checks")
+    n = 10
+    d = device.upload_states(rsim.sample_arenas(world, 0, 1, n))
+    c = device.alloc_words(16 * n)
+    r = device.alloc_words(n)
+    with pytest.raises(rsim.RsimError):
+        device.sim8(d, dev_world, c, r, block=0)
+    with pytest.raises(rsim.RsimError):
+        device.sim8(d, dev_world, c, r, block=257)
+    with pytest.raises(rsim.RsimError):
+        device.sim8(d, dev_world, device.alloc_words(16 * n - 1), r)
+    with pytest.raises(rsim.RsimError):
+        device.sim8(d, dev_world, c, device.alloc_words(n - 1))
+    with pytest.raises(rsim.RsimError):
+        device.sim8(d, dev_world, c, r, trace=device.alloc_words(8))
