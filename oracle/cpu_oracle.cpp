@@ -56,6 +56,13 @@ void rsim_cpu_libm_unary(int fn, uint64_t base, uint64_t n, uint32_t* out) {
     out[i] = fbits(fn == 0 ? crm::sinf_cr(x) : fn == 1 ? crm::cosf_cr(x) : crm::tanf_cr(x));
   }
 }
+// The same over a list of bit patterns.
+void rsim_cpu_libm_unary_list(int fn, const uint32_t* x, uint64_t n, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    float v = ffrom(x[i]);
+    out[i] = fbits(fn == 0 ? crm::sinf_cr(v) : fn == 1 ? crm::cosf_cr(v) : crm::tanf_cr(v));
+  }
+}
 void rsim_cpu_libm_binary(int op, const float* a, const float* b, uint64_t n, uint32_t* out) {
   for (uint64_t i = 0; i < n; ++i) out[i] = fbits(op == 0 ? crm::atan2f_cr(a[i], b[i]) : crm::powf_cr(a[i], b[i]));
 }

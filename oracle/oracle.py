@@ -47,6 +47,7 @@ class Oracle:
         L.rsim_cpu_sim8_aos.argtypes = [_vp, _vp, _vp, _vp, _u32]
         L.rsim_cpu_sim8_batch.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32]
         L.rsim_cpu_libm_unary.argtypes = [ctypes.c_int, _u64, _u64, _vp]
+        L.rsim_cpu_libm_unary_list.argtypes = [ctypes.c_int, _vp, _u64, _vp]
         L.rsim_cpu_libm_binary.argtypes = [ctypes.c_int, _vp, _vp, _u64, _vp]
         L.rsim_cpu_libm_f64.argtypes = [ctypes.c_int, _vp, _vp, _u64, _vp]
         L.rsim_build_world.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
@@ -104,6 +105,12 @@ class Oracle:
     def libm_unary(self, fn, base, n):
         out = np.zeros(n, np.uint32)
         self.L.rsim_cpu_libm_unary(fn, base, n, _p(out))
+        return out
+
+    def libm_unary_list(self, fn, x_bits):
+        x = np.ascontiguousarray(x_bits, np.uint32)
+        out = np.zeros(x.size, np.uint32)
+        self.L.rsim_cpu_libm_unary_list(fn, _p(x), x.size, _p(out))
         return out
 
     def libm_binary(self, op, a, b):

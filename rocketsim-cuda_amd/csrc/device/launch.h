@@ -17,6 +17,7 @@ RS_HD inline size_t aosoa_index(uint32_t a, uint32_t w) {
 }
 
 hipError_t launch_identity(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, bool tick_only);
+hipError_t launch_copy_words(hipStream_t st, const uint32_t* in, uint32_t* out, size_t words);  // words % 4 == 0
 hipError_t launch_phase(hipStream_t st, int k, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status);
 hipError_t launch_step_tick(hipStream_t st, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status);
 hipError_t launch_sim8(hipStream_t st, uint32_t* buf, const StaticData* w, const Controls* ctl, uint32_t* result, uint32_t n,

@@ -6,10 +6,11 @@
 // no lane reads another's arena, so neither the work-group size nor the arena order can change a result.
 //
 // Kernels (reference cuda/sim/sim.cu:8-112):
-//   rs_identity / rs_identity_tick   copy-in and copy-out only (the layout check)
-//   rs_phase                          one phase k of the tick on whole states (per-phase parity)
-//   rs_step_tick                      one full tick at a tick boundary
-//   rs_sim8 / rs_sim8_trace           one decision: 8 ticks, stopping after the scoring tick
+//   k_identity / k_identity_tick   copy-in and copy-out only (the layout check)
+//   k_phase                         one phase k of the tick on whole states (per-phase parity)
+//   k_step_tick                     one full tick at a tick boundary
+//   k_sim8 / k_sim8_trace           one decision: 8 ticks, stopping after the scoring tick
+//   k_copy_words                    a streaming copy of a state buffer (restores between timed runs)
 #include "device/launch.h"
 
 namespace rs {
@@ -104,12 +105,27 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_sim8_trace(uint32_t* buf, const S
   store_tick(buf, a, s);
 }
 
+// A plain streaming copy of whole state buffers (16-byte vectors, grid-stride): the restore between timed runs, and the
+// reference point for measured HBM bytes (it moves exactly 2 x its size).
+__global__ void __launch_bounds__(256) k_copy_words(const uint4* __restrict__ in, uint4* __restrict__ out, size_t n16) {
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) out[i] = in[i];
+}
+
 // ---- launchers ------------------------------------------------------------------------------------------------------------------
 static dim3 grid_of(uint32_t n, uint32_t block) { return dim3((n + block - 1) / block > 0 ? (n + block - 1) / block : 1); }
 
 hipError_t launch_identity(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n, bool tick_only) {
   if (tick_only) k_identity_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(in, out, n);
   else k_identity<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_copy_words(hipStream_t st, const uint32_t* in, uint32_t* out, size_t words) {
+  size_t n16 = words / 4;  // AoSoA buffers are whole blocks of 2784 x 64 words: a multiple of four
+  size_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) blocks = 1;
+  k_copy_words<<<dim3((unsigned)blocks), 256, 0, st>>>(reinterpret_cast<const uint4*>(in), reinterpret_cast<uint4*>(out), n16);
   return hipGetLastError();
 }
 hipError_t launch_phase(hipStream_t st, int k, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status) {

@@ -263,8 +263,8 @@ int rsim_copy_states(rsim_device* dev, rsim_states* dst, const rsim_states* src)
   int rc = use(dev);
   if (rc) return rc;
   if (!dst || !src || dst->n != src->n) return fail(RSIM_ERR_INVALID, "rsim_copy_states: buffers of different sizes");
-  RSIM_HIP(hipMemcpyAsync(dst->d, src->d, rs::dev::aosoa_words(src->n) * 4, hipMemcpyDeviceToDevice, dev->stream),
-           "state copy");
+  if (!src->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_copy_words(dev->stream, src->d, dst->d, rs::dev::aosoa_words(src->n)), "state copy");
   return RSIM_OK;
 }
 int rsim_states_free(rsim_states* s) {

@@ -27,7 +27,10 @@ def oracle():
 @pytest.fixture(scope="session")
 def rsim():
     import rsim
-    rsim.lib()  # raises if librsim.so is not built
+    if not os.path.exists(rsim.LIB_PATH):  # only when nothing is built yet (a fresh checkout)
+        from rsim import build as rb
+        rb.build()
+    rsim.lib()
     return rsim
 
 

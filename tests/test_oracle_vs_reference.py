@@ -1,0 +1,64 @@
+"""The CPU oracle against the reference's own C++ build (oracle/_ref/libref.so, compiled in place from the reference
+sources by `make -C oracle ref`) on fresh seeded inputs: whole states after every tick, every phase, every decision.
+Skipped where the reference tree is absent (the GPU host); the golden vectors carry the same check there."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.skipif(not os.path.exists(orc.REF_LIB), reason="reference build (oracle/_ref) not available")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return orc.Reference()
+
+
+@pytest.mark.parametrize("sc", [0, 1, 2, 3, 4])
+def test_decisions_vs_reference(oracle, ref, sc):
+    world = oracle.build_world(0, 0)
+    n = 48
+    states = oracle.sample_arenas(world, sc, 9000 + sc, n)
+    mine, theirs = states.copy(), states.copy()
+    for d in range(8):
+        ctl = oracle.sample_controls(9100 + sc, d, n)
+        for a in range(n):
+            r1 = oracle.step_ticks(mine[a], world, ctl[a], 8)
+            r2 = ref.step_ticks(theirs[a], world, ctl[a], 8)
+            assert r1 == r2, "scenario %d decision %d arena %d" % (sc, d, a)
+            if not np.array_equal(mine[a], theirs[a]):
+                w = int(np.argmax(mine[a] != theirs[a]))
+                pytest.fail("scenario %d decision %d arena %d: word %d differs" % (sc, d, a, w))
+
+
+@pytest.mark.parametrize("presets", [(1, 3), (2, 5), (4, 6)])
+def test_presets_and_mutators_vs_reference(oracle, ref, presets):
+    rng = np.random.default_rng(presets[0])
+    world = oracle.build_world(presets[0], presets[1], float(rng.uniform(-800, -500)), float(rng.uniform(0.8, 1.2)),
+                               float(rng.uniform(91.25, 100)))
+    states = oracle.sample_arenas(world, 6, 77 + presets[1], 40)
+    mine, theirs = states.copy(), states.copy()
+    for d in range(5):
+        ctl = oracle.sample_controls(78, d, 40)
+        for a in range(40):
+            assert oracle.step_ticks(mine[a], world, ctl[a], 8) == ref.step_ticks(theirs[a], world, ctl[a], 8)
+        assert np.array_equal(mine, theirs), "presets %s decision %d" % (presets, d)
+
+
+def test_phases_vs_reference(oracle, ref):
+    world = oracle.build_world(0, 0)
+    states = oracle.sample_arenas(world, 6, 4321, 32)
+    ctl = oracle.sample_controls(4322, 0, 32)
+    for a in range(32):
+        mine, theirs = states[a].copy(), states[a].copy()
+        for t in range(6):  # several ticks, phase by phase
+            oracle.set_controls(mine, ctl[a])
+            oracle.set_controls(theirs, ctl[a])
+            for k in range(10):
+                assert oracle.phase(k, mine, world) == ref.phase(k, theirs, world)
+                assert np.array_equal(mine, theirs), "arena %d tick %d phase %d" % (a, t, k)
+    s = states[0].copy()
+    assert oracle.phase(10, s, world) == ref.phase(10, s.copy(), world) == -2
+    assert oracle.phase(-1, s, world) == -2
