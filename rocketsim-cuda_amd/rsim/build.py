@@ -17,6 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librsim.so")
+EXAMPLES = {"kickoff": os.path.join(PKG, "examples", "kickoff.cpp")}
 
 ARCH = "gfx950"
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + CSRC, "-I" + INCLUDE]
@@ -62,6 +63,12 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the C++ host examples, linked against librsim.so beside them
+    for name, src in EXAMPLES.items():
+        out = os.path.join(HERE, name)
+        _run(["g++", "-std=c++17", "-O2", "-I" + INCLUDE, src, "-o", out + ".tmp", "-L" + HERE, "-lrsim",
+              "-Wl,-rpath,$ORIGIN"])
+        os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
     return LIB

@@ -1,0 +1,34 @@
+"""The C++ host mirror (rocketsim-cuda_amd/host/sim_device.hpp) end to end: the kickoff example steps replicated kickoffs
+through librsim.so and prints arena 0's ball; the CPU oracle on the same setup must give the same position."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "rocketsim-cuda_amd", "rsim", "kickoff")
+
+
+def test_kickoff_example(rsim, oracle):
+    print("[case] C++ kickoff example, 256 arenas x 6 decisions", flush=True)
+    out = subprocess.run([EXE, "256", "6"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    line = [l for l in out.stdout.splitlines() if l.startswith("arena 0 ball position")][0]
+    got = [float(v) for v in line.split("(")[2].rstrip(")").split(",")]
+    # the same setup on the CPU: the first kickoff seed with the blue car at x = 0, full throttle and boost
+    w = rsim.build_world(0, 0)
+    seed = 0
+    while True:
+        s = rsim.sample_arenas(w, rsim.SCEN_KICKOFF, seed, 1)
+        if s[0, 29:30].view(np.float32)[0] == 0.0:
+            break
+        seed += 1
+    drive = np.array([struct.unpack("<I", struct.pack("<f", 1.0))[0], 0, 0, 0, 0, 0, 1, 0] * 2, np.uint32)
+    ctl = drive.reshape(1, 16)
+    for _ in range(7):  # warm-up decision + 6 timed
+        oracle.sim8(s, w, ctl)
+    ball = s[0, 4:7].view(np.float32)
+    assert ["%.1f" % v for v in ball] == ["%.1f" % v for v in got]
