@@ -44,6 +44,51 @@ void rsim_cpu_sim8_batch(ArenaState* states, const StaticData* w, const Controls
 }
 // Word offset of cars[car].controls in ArenaState (for setting controls before a per-phase run).
 uint64_t rsim_cpu_controls_word(int car) { return (offsetof(ArenaState, cars) + car * sizeof(Car) + offsetof(Car, controls)) / 4; }
+// Brute force: move_cells against the remove-then-push sequence it replaces, on `iters` random sorted lists (lengths up
+// to the cap, so overflow is exercised) and random block moves in the given world's grid. Returns the mismatch count.
+uint64_t rsim_cpu_check_move_cells(const StaticData* w, uint64_t seed, uint64_t iters) {
+  uint64_t bad = 0, x = seed | 1;
+  auto rnd = [&x]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  const uint32_t ncell = w->grid_num_cells[0] * w->grid_num_cells[1] * w->grid_num_cells[2];
+  static ArenaState a, b;
+  for (uint64_t it = 0; it < iters; ++it) {
+    memset(&a, 0, sizeof a);
+    uint32_t n = (uint32_t)(rnd() % (MAX_DYN_CELLS + 1));
+    uint32_t cells[MAX_DYN_CELLS];
+    for (uint32_t i = 0; i < n; ++i) cells[i] = (uint32_t)(rnd() % ncell);
+    for (uint32_t i = 1; i < n; ++i)  // sort (insertion)
+      for (uint32_t j = i; j > 0 && cells[j - 1] > cells[j]; --j) {
+        uint32_t t = cells[j];
+        cells[j] = cells[j - 1];
+        cells[j - 1] = t;
+      }
+    uint32_t h = 21 + (uint32_t)(rnd() % 3);
+    for (uint32_t i = 0; i < n; ++i) a.dyn_cells[i] = DynCell{cells[i], 21 + (uint32_t)(rnd() % 3)};
+    a.n_dyn_cells = n;
+    uint32_t from[3], to[3];
+    for (int k = 0; k < 3; ++k) {
+      from[k] = (uint32_t)(rnd() % w->grid_num_cells[k]);
+      to[k] = (rnd() % 4 == 0) ? from[k] : (uint32_t)(rnd() % w->grid_num_cells[k]);
+    }
+    if (rnd() % 2) {  // make the handle's old block present, as in a real state
+      a.n_dyn_cells = 0;
+      for (uint32_t i = 0; i < n / 2; ++i) dyn_cell_push(a, cells[i], 21 + (uint32_t)(rnd() % 3));
+      update_cells(a, *w, true, h, from);
+    }
+    a.overflow[OVF_DYN_CELLS] = 0;
+    b = a;
+    update_cells(a, *w, false, h, from);
+    update_cells(a, *w, true, h, to);
+    move_cells(b, *w, h, from, to);
+    if (memcmp(&a, &b, sizeof a) != 0) ++bad;
+  }
+  return bad;
+}
 uint64_t rsim_cpu_arena_words() { return ARENA_WORDS; }
 uint64_t rsim_cpu_tick_words() { return TICK_WORDS; }
 uint64_t rsim_cpu_static_words() { return STATIC_WORDS; }

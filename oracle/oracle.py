@@ -53,6 +53,8 @@ class Oracle:
         L.rsim_build_world.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
         L.rsim_sample_arenas.argtypes = [_vp, ctypes.c_int, _u64, _u32, _u32, _vp]
         L.rsim_sample_controls.argtypes = [_u64, _u32, _u32, _u32, _vp]
+        L.rsim_cpu_check_move_cells.argtypes = [_vp, _u64, _u64]
+        L.rsim_cpu_check_move_cells.restype = _u64
         for f in ("rsim_cpu_arena_words", "rsim_cpu_tick_words", "rsim_cpu_static_words", "rsim_cpu_controls_word"):
             getattr(L, f).restype = _u64
         self.L = L
@@ -84,6 +86,10 @@ class Oracle:
         assert states.flags.c_contiguous and controls.flags.c_contiguous
         self.L.rsim_cpu_sim8_batch(_p(states), _p(world), _p(controls), _p(res), n, threads)
         return res[:n]
+
+    def check_move_cells(self, world, seed, iters):
+        """Mismatches of the one-pass cell move against remove-then-push on random lists (brute force)."""
+        return int(self.L.rsim_cpu_check_move_cells(_p(world), seed, iters))
 
     # -- workloads (the same builder source as the product's) --
     def build_world(self, p0=0, p1=0, gravity_z=-650.0, boost_accel_scale=1.0, ball_radius=91.25):

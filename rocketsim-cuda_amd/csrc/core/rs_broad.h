@@ -26,10 +26,18 @@ RS_FN Aabb box_from_half_extents(V3 half, float margin, const Aff& t) {  // shar
 }
 
 // ---- the pair array (overlapping_pair_cache.cuh) -----------------------------------------------------------------------------
+// The first pair whose proxies carry the unique ids (id0, id1) (overlapping_pair_cache.cuh find_pair). The pair words
+// are loaded for all slots at once and the first match selected, instead of one dependent load chain per slot.
 RS_FN int find_pair(const ArenaState& s, const StaticData& w, uint32_t id0, uint32_t id1) {
-  for (uint32_t i = 0; i < s.n_pairs; ++i)
-    if (w.proxies[s.pairs[i].proxy0].unique_id == id0 && w.proxies[s.pairs[i].proxy1].unique_id == id1) return (int)i;
-  return -1;
+  const uint32_t n = s.n_pairs < (uint32_t)MAX_PAIRS ? s.n_pairs : (uint32_t)MAX_PAIRS;
+  Pair p[MAX_PAIRS];
+  RS_UNROLL
+  for (int i = 0; i < MAX_PAIRS; ++i) p[i] = (uint32_t)i < n ? s.pairs[i] : Pair{0, 0};
+  int found = -1;
+  RS_UNROLL
+  for (int i = MAX_PAIRS - 1; i >= 0; --i)
+    if ((uint32_t)i < n && w.proxies[p[i].proxy0].unique_id == id0 && w.proxies[p[i].proxy1].unique_id == id1) found = i;
+  return found;
 }
 RS_FN void add_pair(ArenaState& s, const StaticData& w, uint32_t h0, uint32_t h1) {
   const ProxyConst &p0 = w.proxies[h0], &p1 = w.proxies[h1];
@@ -46,13 +54,20 @@ RS_FN void add_pair(ArenaState& s, const StaticData& w, uint32_t h0, uint32_t h1
 }
 // collision_dispatcher.rs release_manifold (collision_dispatcher_release.cuh:10-39): the manifold of the unordered body
 // pair leaves the pool by swap-remove; the active list drops it and renumbers the moved one.
-RS_FN int find_manifold(const ArenaState& s, uint32_t b0, uint32_t b1) {
+RS_FN int find_manifold(const ArenaState& s, uint32_t b0, uint32_t b1) {  // the first match, all slots loaded at once
   uint32_t lo = b0 < b1 ? b0 : b1, hi = b0 < b1 ? b1 : b0;
-  for (uint32_t i = 0; i < s.n_manifolds; ++i) {
-    uint32_t a = s.manifolds[i].body0, b = s.manifolds[i].body1;
-    if ((a < b ? a : b) == lo && (a < b ? b : a) == hi) return (int)i;
+  const uint32_t n = s.n_manifolds < (uint32_t)MAX_MANIFOLDS ? s.n_manifolds : (uint32_t)MAX_MANIFOLDS;
+  uint32_t a[MAX_MANIFOLDS], b[MAX_MANIFOLDS];
+  RS_UNROLL
+  for (int i = 0; i < MAX_MANIFOLDS; ++i) {
+    a[i] = (uint32_t)i < n ? s.manifolds[i].body0 : 0u;
+    b[i] = (uint32_t)i < n ? s.manifolds[i].body1 : 0u;
   }
-  return -1;
+  int found = -1;
+  RS_UNROLL
+  for (int i = MAX_MANIFOLDS - 1; i >= 0; --i)
+    if ((uint32_t)i < n && (a[i] < b[i] ? a[i] : b[i]) == lo && (a[i] < b[i] ? b[i] : a[i]) == hi) found = i;
+  return found;
 }
 RS_FN void release_manifold(ArenaState& s, uint32_t b0, uint32_t b1) {
   int f = find_manifold(s, b0, b1);
@@ -229,6 +244,73 @@ RS_FN void update_cells(ArenaState& s, const StaticData& w, bool add, uint32_t h
         else dyn_cell_remove(s, cell, h);
       }
 }
+// The 3x3x3 block of cells around idx, clipped to the grid (update_cells' ranges).
+RS_FN void cell_block(const StaticData& w, const uint32_t idx[3], uint32_t lo[3], uint32_t n[3]) {
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = (idx[a] > 1 ? idx[a] : 1) - 1;
+    uint32_t top = w.grid_num_cells[a] - 1, p = idx[a] + 1;
+    n[a] = (top < p ? top : p) - lo[a] + 1;
+  }
+}
+// Values v of [lo, lo + n) with v < x.
+RS_FN uint32_t count_below(uint32_t x, uint32_t lo, uint32_t n) { return x <= lo ? 0u : (x - lo < n ? x - lo : n); }
+
+// A dynamic handle's move from the cell block around `from` to the block around `to`: the same list as removing (cell, h)
+// for every cell of the old block (first match each) and then pushing (cell, h) for every cell of the new block in
+// ascending order (each after the last entry of a cell <= it; pushes past MAX_DYN_CELLS count as overflow), computed in
+// one pass. The list is read once into registers and every entry written once to its final slot: the remove/push loops
+// shift the list through memory once per cell, a chain of dependent accesses a wave pays whenever any of its lanes moves.
+RS_FN void move_cells(ArenaState& s, const StaticData& w, uint32_t h, const uint32_t from[3], const uint32_t to[3]) {
+  const uint32_t nz = w.grid_num_cells[2], nyz = w.grid_num_cells[1] * nz;
+  uint32_t olo[3], on[3], nlo[3], nn[3];
+  cell_block(w, from, olo, on);
+  cell_block(w, to, nlo, nn);
+  const uint32_t total = nn[0] * nn[1] * nn[2], plane = nn[1] * nn[2];
+  const uint32_t n = s.n_dyn_cells < (uint32_t)MAX_DYN_CELLS ? s.n_dyn_cells : (uint32_t)MAX_DYN_CELLS;
+  DynCell e[MAX_DYN_CELLS];
+  bool keep[MAX_DYN_CELLS];
+  uint32_t removed = 0, kept = 0;
+RS_UNROLL
+  for (int i = 0; i < MAX_DYN_CELLS; ++i) e[i] = (uint32_t)i < n ? s.dyn_cells[i] : DynCell{0, 0};
+RS_UNROLL
+  for (int i = 0; i < MAX_DYN_CELLS; ++i) {
+    uint32_t c = e[i].cell, ci = c / nyz, cj = (c - ci * nyz) / nz, ck = c - ci * nyz - cj * nz;
+    uint32_t di = ci - olo[0], dj = cj - olo[1], dk = ck - olo[2];  // unsigned: out of the block wraps high
+    bool in_old = e[i].handle == h && di < on[0] && dj < on[1] && dk < on[2];
+    uint32_t bit = in_old ? 1u << ((di * 3 + dj) * 3 + dk) : 0u;
+    bool drop = in_old && (removed & bit) == 0;
+    if (drop) removed |= bit;
+    keep[i] = (uint32_t)i < n && !drop;
+    kept += keep[i] ? 1u : 0u;
+  }
+  const uint32_t room = (uint32_t)MAX_DYN_CELLS - kept, placed = total < room ? total : room;
+  uint32_t m = 0, done = 0;  // old entries written, new entries written
+RS_UNROLL
+  for (int i = 0; i < MAX_DYN_CELLS; ++i) {
+    if (!keep[i]) continue;
+    // new cells below this entry's cell (new cells are visited in ascending index order, which is (i, j, k) order)
+    uint32_t c = e[i].cell, ci = c / nyz, cj = (c - ci * nyz) / nz, ck = c - ci * nyz - cj * nz;
+    uint32_t below = count_below(ci, nlo[0], nn[0]) * plane;
+    if (ci - nlo[0] < nn[0]) {
+      below += count_below(cj, nlo[1], nn[1]) * nn[2];
+      if (cj - nlo[1] < nn[1]) below += count_below(ck, nlo[2], nn[2]);
+    }
+    if (below > placed) below = placed;
+    for (; done < below; ++done) {
+      uint32_t a = done / plane, r = done - a * plane, b = r / nn[2], k2 = r - b * nn[2];
+      s.dyn_cells[done + m] = DynCell{(nlo[0] + a) * nyz + (nlo[1] + b) * nz + nlo[2] + k2, h};
+    }
+    s.dyn_cells[done + m] = e[i];
+    ++m;
+  }
+  for (; done < placed; ++done) {
+    uint32_t a = done / plane, r = done - a * plane, b = r / nn[2], k2 = r - b * nn[2];
+    s.dyn_cells[done + m] = DynCell{(nlo[0] + a) * nyz + (nlo[1] + b) * nz + nlo[2] + k2, h};
+  }
+  for (uint32_t i = kept + placed; i < n; ++i) s.dyn_cells[i] = DynCell{0, 0};
+  s.n_dyn_cells = kept + placed;
+  s.overflow[OVF_DYN_CELLS] += total - placed;
+}
 RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb box) {  // grid_broadphase.cuh:68-84
   uint32_t h = w.dyn_handle[d];
   s.dyn_proxy[d].aabb = box;
@@ -241,9 +323,9 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
   uint32_t nidx = cell_index(w, ni);
   s.dyn_proxy[d].cell_idx = nidx;
   if (nidx != old) {
-    update_cells(s, w, false, h, s.dyn_proxy[d].indices);
+    uint32_t from[3] = {s.dyn_proxy[d].indices[0], s.dyn_proxy[d].indices[1], s.dyn_proxy[d].indices[2]};
     for (int a = 0; a < 3; ++a) s.dyn_proxy[d].indices[a] = ni[a];
-    update_cells(s, w, true, h, ni);
+    move_cells(s, w, h, from, ni);  // = update_cells(remove, from) then update_cells(add, ni)
   }
 }
 

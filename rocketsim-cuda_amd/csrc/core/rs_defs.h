@@ -26,6 +26,9 @@
 
 #if defined(__clang__)
 #pragma clang fp contract(off)
+#define RS_UNROLL _Pragma("unroll")
+#else
+#define RS_UNROLL
 #endif
 
 #if !defined(__HIP_DEVICE_COMPILE__) && (defined(__FMA__) || defined(__FP_FAST_FMAF))

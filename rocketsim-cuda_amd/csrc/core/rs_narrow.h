@@ -377,18 +377,16 @@ RS_FN bool convex_plane(Np& c, uint32_t obj, uint32_t plane_obj, const Aff& wt, 
 
 // ---- the BVH4 cursor (bvh.cuh:31-56) -----------------------------------------------------------------------------------------
 struct BvhCursor {
-  uint32_t stack[128];
-  int n;
+  TravStack st;
 };
 RS_FN void cursor_init(BvhCursor& cur, const MeshInfo& mesh, const Aabb& box) {
-  cur.n = 0;
+  cur.st.n = 0;
   if (!overlaps(box, mesh.root_aabb)) return;
-  cur.stack[0] = 0;
-  cur.n = 1;
+  cur.st.push(0);
 }
 RS_FN bool cursor_next(BvhCursor& cur, const StaticData& w, const MeshInfo& mesh, const Aabb& box, uint32_t* tri) {
-  while (cur.n != 0) {
-    uint32_t work = cur.stack[--cur.n];
+  while (cur.st.n != 0) {
+    uint32_t work = cur.st.pop();
     if (work & BVH_LEAF) {
       *tri = w.leaves[mesh.leaf_first + (work & ~BVH_LEAF)].tri;
       return true;
@@ -396,7 +394,7 @@ RS_FN bool cursor_next(BvhCursor& cur, const StaticData& w, const MeshInfo& mesh
     const WideNode& node = w.nodes[mesh.node_first + work];
     uint32_t mask = node_mask(node, box);
     for (int lane = (int)node.child_count - 1; lane >= 0; --lane)
-      if ((mask & (1u << lane)) && cur.n < 128) cur.stack[cur.n++] = node.children[lane];
+      if (mask & (1u << lane)) cur.st.push(node.children[lane]);
   }
   return false;
 }

@@ -3,6 +3,7 @@
 // bvh.cuh). Candidate order and tie rules decide which body wins a tie, so they follow the reference exactly.
 #pragma once
 #include "rs_body.h"
+#include "rs_stack.h"
 
 namespace rs {
 
@@ -417,11 +418,11 @@ RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, Ra
   if (!overlaps(rp.box, mesh.root_aabb)) return;
   float org[3][4], inv[3][4];
   packet_pos_dir(rp, org, inv);
-  uint32_t stack[128];
-  stack[0] = 0;
-  int sp = 1;
-  while (sp != 0) {
-    uint32_t work = stack[--sp];
+  TravStack st;
+  st.n = 0;
+  st.push(0);
+  while (st.n != 0) {
+    uint32_t work = st.pop();
     if (work & BVH_LEAF) {
       const Leaf& leaf = w.leaves[mesh.leaf_first + (work & ~BVH_LEAF)];
       uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
@@ -435,7 +436,7 @@ RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, Ra
     const WideNode& node = w.nodes[mesh.node_first + work];
     uint32_t mask = node_mask(node, rp.box);
     for (int lane = (int)node.child_count - 1; lane >= 0; --lane)
-      if ((mask & (1u << lane)) && sp < 128) stack[sp++] = node.children[lane];
+      if (mask & (1u << lane)) st.push(node.children[lane]);
   }
 }
 
@@ -534,8 +535,16 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       seen |= 1u << h;
       if (overlaps(rb, proxy_box(s, w, h))) ray_candidate(s, w, h, hits);
     }
-    for (uint32_t e = 0; e < s.n_dyn_cells; ++e) {
-      if (s.dyn_cells[e].cell != cell) continue;
+    // the cell's dynamic entries: the list is sorted by cell (an invariant of the format, dump.rs:683-685), so its run
+    // is found by bisection and walked in list order, the entries a full scan would visit, in the same order
+    uint32_t lo = 0, hi = s.n_dyn_cells < (uint32_t)MAX_DYN_CELLS ? s.n_dyn_cells : (uint32_t)MAX_DYN_CELLS;
+    const uint32_t end = hi;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (s.dyn_cells[mid].cell < cell) lo = mid + 1;
+      else hi = mid;
+    }
+    for (uint32_t e = lo; e < end && s.dyn_cells[e].cell == cell; ++e) {
       uint32_t h = s.dyn_cells[e].handle;
       if (seen & (1u << h)) continue;
       seen |= 1u << h;

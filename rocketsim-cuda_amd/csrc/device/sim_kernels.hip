@@ -129,17 +129,18 @@ hipError_t launch_copy_words(hipStream_t st, const uint32_t* in, uint32_t* out, 
   return hipGetLastError();
 }
 hipError_t launch_phase(hipStream_t st, int k, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status) {
-  k_phase<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(k, buf, w, n, status);
+  k_phase<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, trav_lds_bytes(SIM8_BLOCK), st>>>(k, buf, w, n, status);
   return hipGetLastError();
 }
 hipError_t launch_step_tick(hipStream_t st, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status) {
-  k_step_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, 0, st>>>(buf, w, n, status);
+  k_step_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, trav_lds_bytes(SIM8_BLOCK), st>>>(buf, w, n, status);
   return hipGetLastError();
 }
 hipError_t launch_sim8(hipStream_t st, uint32_t* buf, const StaticData* w, const Controls* ctl, uint32_t* result, uint32_t n,
                        uint32_t block, uint32_t* trace) {
-  if (trace) k_sim8_trace<<<grid_of(n, block), block, 0, st>>>(buf, w, ctl, result, n, trace);
-  else k_sim8<<<grid_of(n, block), block, 0, st>>>(buf, w, ctl, result, n);
+  const size_t lds = trav_lds_bytes(block);  // the BVH walks' traversal stacks
+  if (trace) k_sim8_trace<<<grid_of(n, block), block, lds, st>>>(buf, w, ctl, result, n, trace);
+  else k_sim8<<<grid_of(n, block), block, lds, st>>>(buf, w, ctl, result, n);
   return hipGetLastError();
 }
 
