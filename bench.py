@@ -118,6 +118,20 @@ def main():
 
     results = [dev.read_words(r) for r in d_res]
     final = dev.download_states(d_work)
+
+    # the same K decisions in one launch (rsim_run: each arena held on chip across decisions), checked against the plain run
+    print("[bench] the same decisions in one launch", file=sys.stderr, flush=True)
+    d_ctl_all = dev.upload_words(np.concatenate(controls[:K]))
+    d_res_all = dev.alloc_words(K * n)
+    dev.copy_states(d_work, d_init)
+    dev.sync()
+    t1 = time.perf_counter()
+    dev.timer_begin()
+    dev.run(d_work, dw, d_ctl_all, d_res_all, K, block=args.block)
+    run_ms = dev.timer_end()
+    run_wall = time.perf_counter() - t1
+    run_same = bool(np.array_equal(dev.download_states(d_work)[:, :2567], final[:, :2567]) and
+                    np.array_equal(dev.read_words(d_res_all).reshape(K, n), np.stack(results)))
     stepped = sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
     nominal = n * 8 * K
     flags = sum(int(((r & 0xC0000000) != 0).sum()) for r in results)
@@ -162,6 +176,8 @@ def main():
         "arena_ticks_nominal": nominal,
         "nominal_arena_ticks_per_s": nominal / wall,
         "flagged_arenas": flags,
+        "multi_decision_launch": {"arena_ticks_per_s": stepped / run_wall, "kernel_ms": run_ms, "kernel": "k_run",
+                                  "identical_to_plain": run_same},
         "word_diffs": diffs,
     }
     print(json.dumps(line), flush=True)

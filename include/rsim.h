@@ -38,6 +38,7 @@ extern "C" {
 #define RSIM_SIM8_SCORED_MASK 0xFFu         /* the tick 1..8 after which the ball was first scored, 0 for none */
 #define RSIM_SIM8_UNIMPLEMENTED 0x80000000u /* a tick up to then reached a shape pair V3's port does not implement */
 #define RSIM_SIM8_OVERFLOW 0x40000000u      /* an overflow counter of the state written back is nonzero */
+#define RSIM_SIM8_BAD_VARIANT 0x20000000u   /* (rsim_sim8_variants) the arena's world index is past the table: not stepped */
 #define RSIM_SIM8_BLOCK 64                  /* work-group size in production: one wave64 per AoSoA-64 block */
 #define RSIM_MAX_BLOCK 256                  /* largest work-group size the tick kernels accept */
 #define RSIM_NUM_PHASES 10                  /* P0, P1, P2, P3a, P3b, P4, P5, P6, P7, P8 (phase.cuh:17-46) */
@@ -69,6 +70,11 @@ void* rsim_stream(rsim_device* dev);
 
 /* upload_static (device.rs:69-71): one StaticData (rsim_static_words() words). */
 int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** world);
+/* No counterpart in the reference (one StaticData per launch, device.rs:155): a table of `count` StaticData back to back
+ * (count <= RSIM_MAX_WORLDS), for batches that mix car pairings or mutators; rsim_sim8_variants picks one per arena. */
+#define RSIM_MAX_WORLDS 1024
+int rsim_upload_worlds(rsim_device* dev, const void* static_data, uint32_t count, rsim_world** world);
+uint32_t rsim_world_count(const rsim_world* world);
 int rsim_world_free(rsim_world* world);
 
 /* upload_states / alloc_states (device.rs:73-80): n ArenaStates packed to AoSoA-64 on the host and uploaded; alloc zeroes. */
@@ -102,6 +108,19 @@ void* rsim_words_device_ptr(rsim_words* buf);
  * written). */
 int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
               uint32_t block, rsim_words* trace);
+/* sim8 over a world table: arena a steps in world variant[a] (variant: >= n words, or null for world 0 everywhere). An
+ * index >= rsim_world_count(world) leaves the arena untouched and sets RSIM_SIM8_BAD_VARIANT in its result word. No
+ * counterpart in the reference; with a one-world table and a null variant it is rsim_sim8. */
+int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                       const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace);
+
+/* `decisions` consecutive decisions in one launch, asynchronous: the same results as that many rsim_sim8_variants calls
+ * (decision d + 1 starts from decision d's written-back state), with each arena held on chip from the first decision to
+ * the last instead of being copied out and in between. controls: decisions x n x 16 words (decision-major); results:
+ * decisions x n words. variant: as in rsim_sim8_variants (null: world 0). No counterpart in the reference; for callers
+ * whose controls are known ahead (replays, scripted opponents, benchmarks). */
+int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant, const rsim_words* controls,
+             rsim_words* results, uint32_t decisions, uint32_t block);
 
 /* The debug and parity kernels (device.rs:90-141), synchronous. status_out: n ints (PhaseStatus per arena) or null. */
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out);      /* copy-in, copy-out */

@@ -440,6 +440,22 @@ RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, Ra
   }
 }
 
+// The first mesh (plane) record at or after `from` that belongs to `body`; NUM_MESHES (NUM_PLANES) when none does.
+RS_FN int next_mesh_of(const StaticData& w, uint32_t body, int from) {
+  int found = NUM_MESHES;
+  RS_UNROLL
+  for (int mi = NUM_MESHES - 1; mi >= 0; --mi)
+    if (mi >= from && w.meshes[mi].body == body) found = mi;
+  return found;
+}
+RS_FN int next_plane_of(const StaticData& w, uint32_t body, int from) {
+  int found = NUM_PLANES;
+  RS_UNROLL
+  for (int p = NUM_PLANES - 1; p >= 0; --p)
+    if (p >= from && w.planes[p].body == body) found = p;
+  return found;
+}
+
 // collision_world.rs quad_ray_test (raycaster.cuh:20-70): the rays into the body's frame, then its shape
 RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, ClosestHits& h) {
   BodyView co = body_view(s, w, body);
@@ -478,11 +494,11 @@ RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, Cl
   } else if (bc.shape_kind == k::SHAPE_SPHERE) {
     sphere_rays(bc.sphere_margin, br, rp);
   } else if (bc.shape_kind == k::SHAPE_PLANE) {
-    for (int p = 0; p < NUM_PLANES; ++p)
-      if (w.planes[p].body == body) plane_rays(w.planes[p], br, rp);
+    for (int p = next_plane_of(w, body, 0); p < NUM_PLANES; p = next_plane_of(w, body, p + 1)) plane_rays(w.planes[p], br, rp);
   } else {
-    for (int mi = 0; mi < NUM_MESHES; ++mi)
-      if (w.meshes[mi].body == body) mesh_rays(w, w.meshes[mi], br, rp);
+    // every mesh record of the body, in record order; one call site, so lanes walking different meshes walk together
+    for (int mi = next_mesh_of(w, body, 0); mi < NUM_MESHES; mi = next_mesh_of(w, body, mi + 1))
+      mesh_rays(w, w.meshes[mi], br, rp);
   }
 }
 

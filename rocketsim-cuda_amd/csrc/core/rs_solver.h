@@ -239,11 +239,16 @@ RS_FN Row friction_row(int ia, int ib, const SolverBody& sa, const SolverBody& s
   r.rhs = fneg(v1n + v2n) * r.jinv;
   return r;
 }
-RS_FN float resolve_split(Row& r, SolverBody& a, SolverBody& b) {  // :518-565
+// The row solves read and write the bodies' deltas through the bank (fields: 0 dlin, 1 dang, 2 push, 3 turn); the
+// read-only body constants (inverse mass, whether it is a dynamic body) come from the SolverBody.
+enum BankField { BF_DLIN = 0, BF_DANG = 1, BF_PUSH = 2, BF_TURN = 3 };
+RS_FN float resolve_split(Row& r, const SolverBody& a, const SolverBody& b, DeltaBank& bank) {  // :518-565
   if (r.rhs_pen == 0.0f) return 0.0f;
   float di = r.rhs_pen;
-  float d1 = a.has_orig ? bullet_dot(r.n1, a.push) + bullet_dot(r.r1xn, a.turn) : 0.0f;
-  float d2 = b.has_orig ? bullet_dot(r.n2, b.push) + bullet_dot(r.r2xn, b.turn) : 0.0f;
+  V3 pa = a.has_orig ? bank.get(r.a, BF_PUSH) : splat(0.0f), ta = a.has_orig ? bank.get(r.a, BF_TURN) : splat(0.0f);
+  V3 pb = b.has_orig ? bank.get(r.b, BF_PUSH) : splat(0.0f), tb = b.has_orig ? bank.get(r.b, BF_TURN) : splat(0.0f);
+  float d1 = a.has_orig ? bullet_dot(r.n1, pa) + bullet_dot(r.r1xn, ta) : 0.0f;
+  float d2 = b.has_orig ? bullet_dot(r.n2, pb) + bullet_dot(r.r2xn, tb) : 0.0f;
   di = di - d1 * r.jinv;
   di = di - d2 * r.jinv;
   float sum = r.applied_push + di;
@@ -254,19 +259,21 @@ RS_FN float resolve_split(Row& r, SolverBody& a, SolverBody& b) {  // :518-565
     r.applied_push = sum;
   }
   if (a.has_orig) {
-    a.push = a.push + r.n1 * a.inv_mass * di;
-    a.turn = a.turn + r.ang_a * di;
+    bank.set(r.a, BF_PUSH, pa + r.n1 * a.inv_mass * di);
+    bank.set(r.a, BF_TURN, ta + r.ang_a * di);
   }
-  if (b.has_orig) {
-    b.push = b.push + r.n2 * b.inv_mass * di;
-    b.turn = b.turn + r.ang_b * di;
+  if (b.has_orig) {  // re-read: a and b may be the same body
+    bank.set(r.b, BF_PUSH, bank.get(r.b, BF_PUSH) + r.n2 * b.inv_mass * di);
+    bank.set(r.b, BF_TURN, bank.get(r.b, BF_TURN) + r.ang_b * di);
   }
   return di / r.jinv;
 }
-RS_FN float resolve_row(Row& r, SolverBody& a, SolverBody& b, bool two_sided) {  // :427-516
+RS_FN float resolve_row(Row& r, const SolverBody& a, const SolverBody& b, DeltaBank& bank, bool two_sided) {  // :427-516
   float di = r.rhs;
-  float d1 = a.has_orig ? bullet_dot(r.n1, a.dlin) + bullet_dot(r.r1xn, a.dang) : 0.0f;
-  float d2 = b.has_orig ? bullet_dot(r.n2, b.dlin) + bullet_dot(r.r2xn, b.dang) : 0.0f;
+  V3 la = a.has_orig ? bank.get(r.a, BF_DLIN) : splat(0.0f), aa = a.has_orig ? bank.get(r.a, BF_DANG) : splat(0.0f);
+  V3 lb = b.has_orig ? bank.get(r.b, BF_DLIN) : splat(0.0f), ab = b.has_orig ? bank.get(r.b, BF_DANG) : splat(0.0f);
+  float d1 = a.has_orig ? bullet_dot(r.n1, la) + bullet_dot(r.r1xn, aa) : 0.0f;
+  float d2 = b.has_orig ? bullet_dot(r.n2, lb) + bullet_dot(r.r2xn, ab) : 0.0f;
   di = di - d1 * r.jinv;
   di = di - d2 * r.jinv;
   float sum = r.applied + di;
@@ -280,12 +287,12 @@ RS_FN float resolve_row(Row& r, SolverBody& a, SolverBody& b, bool two_sided) { 
     r.applied = sum;
   }
   if (a.has_orig) {
-    a.dlin = a.dlin + r.n1 * a.inv_mass * di;
-    a.dang = a.dang + r.ang_a * di;
+    bank.set(r.a, BF_DLIN, la + r.n1 * a.inv_mass * di);
+    bank.set(r.a, BF_DANG, aa + r.ang_a * di);
   }
-  if (b.has_orig) {
-    b.dlin = b.dlin + r.n2 * b.inv_mass * di;
-    b.dang = b.dang + r.ang_b * di;
+  if (b.has_orig) {  // re-read: a and b may be the same body
+    bank.set(r.b, BF_DLIN, bank.get(r.b, BF_DLIN) + r.n2 * b.inv_mass * di);
+    bank.set(r.b, BF_DANG, bank.get(r.b, BF_DANG) + r.ang_b * di);
   }
   return di / r.jinv;
 }
@@ -413,6 +420,13 @@ RS_FN void solver_setup(Solver& sv, ArenaState& s, const StaticData& w, float dt
   }
 }
 RS_FN void solver_iterate(Solver& sv) {  // :196-251
+  DeltaBank bank;
+  for (int i = 0; i < sv.n_bodies && i < BANK_BODIES; ++i) {
+    bank.set(i, BF_DLIN, sv.bodies[i].dlin);
+    bank.set(i, BF_DANG, sv.bodies[i].dang);
+    bank.set(i, BF_PUSH, sv.bodies[i].push);
+    bank.set(i, BF_TURN, sv.bodies[i].turn);
+  }
   bool run[MAX_ROWS];
   int remaining = sv.n_contact;
   for (int i = 0; i < sv.n_contact; ++i) run[i] = true;
@@ -420,7 +434,7 @@ RS_FN void solver_iterate(Solver& sv) {  // :196-251
     for (int i = 0; i < sv.n_contact; ++i) {
       if (!run[i]) continue;
       Row& r = sv.contact[i];
-      float res = resolve_split(r, sv.bodies[r.a], sv.bodies[r.b]);
+      float res = resolve_split(r, sv.bodies[r.a], sv.bodies[r.b], bank);
       if (res * res == 0.0f) {
         run[i] = false;
         --remaining;
@@ -432,7 +446,7 @@ RS_FN void solver_iterate(Solver& sv) {  // :196-251
     for (int i = 0; i < sv.n_contact; ++i) {
       Row& r = sv.contact[i];
       if (r.split_only) continue;
-      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], false);
+      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], bank, false);
       lsr = fmax_rs(res * res, lsr);
     }
     for (int i = 0; i < sv.n_friction; ++i) {
@@ -442,10 +456,16 @@ RS_FN void solver_iterate(Solver& sv) {  // :196-251
       float lim = r.friction * total;
       r.lower = fneg(lim);
       r.upper = lim;
-      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], true);
+      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], bank, true);
       lsr = fmax_rs(res * res, lsr);
     }
     if (lsr == 0.0f) break;
+  }
+  for (int i = 0; i < sv.n_bodies && i < BANK_BODIES; ++i) {
+    sv.bodies[i].dlin = bank.get(i, BF_DLIN);
+    sv.bodies[i].dang = bank.get(i, BF_DANG);
+    sv.bodies[i].push = bank.get(i, BF_PUSH);
+    sv.bodies[i].turn = bank.get(i, BF_TURN);
   }
 }
 RS_FN void solver_finish(Solver& sv, ArenaState& s, float dt) {  // :271-301

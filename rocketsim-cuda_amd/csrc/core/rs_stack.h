@@ -6,13 +6,19 @@
 // deeper entries spill to private memory. A pop is then an LDS read instead of a scratch round trip, which is what the
 // walk's node-by-node loop waits on. Kernels that step arenas launch with trav_lds_bytes(block) of dynamic LDS.
 #pragma once
-#include "rs_defs.h"
+#include "rs_vec.h"
 
 namespace rs {
 
 constexpr int TRAV_CAP = 128;
 constexpr int TRAV_LDS = 32;
-RS_HD inline size_t trav_lds_bytes(uint32_t block) { return (size_t)TRAV_LDS * block * sizeof(uint32_t); }
+// The solver's delta bank (below): the mutable velocity and split-impulse deltas of up to 8 solver bodies.
+constexpr int BANK_BODIES = 8;
+constexpr int BANK_WORDS = BANK_BODIES * 12;
+// Both live in the same per-lane LDS slice (they are used in different phases), entry-major: word k of lane t of the
+// work-group is at k * blockDim + t.
+constexpr int LANE_LDS_WORDS = TRAV_LDS > BANK_WORDS ? TRAV_LDS : BANK_WORDS;
+RS_HD inline size_t trav_lds_bytes(uint32_t block) { return (size_t)LANE_LDS_WORDS * block * sizeof(uint32_t); }
 
 #if defined(__HIP_DEVICE_COMPILE__)
 extern __shared__ uint32_t rs_trav_lds[];
@@ -30,7 +36,26 @@ struct TravStack {
     return n < TRAV_LDS ? rs_trav_lds[(uint32_t)n * blockDim.x + threadIdx.x] : spill[n - TRAV_LDS];
   }
 };
+// The solver bodies' deltas (dlin, dang, push, turn) during the iterations: in LDS on the GPU, so that a row reading a
+// body the previous row just updated waits on LDS rather than on a private-memory round trip.
+struct DeltaBank {
+  __device__ V3 get(int body, int field) const {
+    const float* p = reinterpret_cast<const float*>(rs_trav_lds) + ((uint32_t)(body * 12 + field * 3) * blockDim.x + threadIdx.x);
+    return V3{p[0], p[blockDim.x], p[2 * blockDim.x]};
+  }
+  __device__ void set(int body, int field, V3 v) {
+    float* p = reinterpret_cast<float*>(rs_trav_lds) + ((uint32_t)(body * 12 + field * 3) * blockDim.x + threadIdx.x);
+    p[0] = v.x;
+    p[blockDim.x] = v.y;
+    p[2 * blockDim.x] = v.z;
+  }
+};
 #else
+struct DeltaBank {
+  V3 v[BANK_BODIES][4];
+  V3 get(int body, int field) const { return v[body][field]; }
+  void set(int body, int field, V3 x) { v[body][field] = x; }
+};
 struct TravStack {
   uint32_t a[TRAV_CAP];
   int n;

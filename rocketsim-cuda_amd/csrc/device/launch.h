@@ -20,8 +20,21 @@ hipError_t launch_identity(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 hipError_t launch_copy_words(hipStream_t st, const uint32_t* in, uint32_t* out, size_t words);  // words % 4 == 0
 hipError_t launch_phase(hipStream_t st, int k, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status);
 hipError_t launch_step_tick(hipStream_t st, uint32_t* buf, const StaticData* w, uint32_t n, int32_t* status);
-hipError_t launch_sim8(hipStream_t st, uint32_t* buf, const StaticData* w, const Controls* ctl, uint32_t* result, uint32_t n,
-                       uint32_t block, uint32_t* trace);
+// The result-word bit of an arena whose variant index is past the world table (the arena is not stepped).
+constexpr uint32_t SIM8_BAD_VARIANT = 0x20000000u;
+
+struct Sim8Args {
+  uint32_t* buf;             // AoSoA-64 states, n arenas, stepped in place
+  const StaticData* worlds;  // nworlds StaticData back to back
+  uint32_t nworlds;
+  const uint32_t* variant;   // null, or n world indices
+  const Controls* ctl;       // Controls[n][NUM_CARS]
+  uint32_t* result;          // n result words
+  uint32_t n;
+  uint32_t* trace;           // null, or TICKS_PER_DECISION slabs of aosoa_words(n)
+};
+hipError_t launch_sim8(hipStream_t st, const Sim8Args& p, uint32_t block);
+hipError_t launch_run(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block);  // ctl/result: per decision
 
 }  // namespace dev
 }  // namespace rs

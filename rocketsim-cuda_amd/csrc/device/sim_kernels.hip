@@ -10,6 +10,7 @@
 //   k_phase                         one phase k of the tick on whole states (per-phase parity)
 //   k_step_tick                     one full tick at a tick boundary
 //   k_sim8 / k_sim8_trace           one decision: 8 ticks, stopping after the scoring tick
+//   k_run                           several decisions per launch, the arena held in the lane from first to last
 //   k_copy_words                    a streaming copy of a state buffer (restores between timed runs)
 #include "device/launch.h"
 
@@ -81,28 +82,52 @@ struct TraceOut {  // the tick-boundary words after tick t into slab t - 1 of th
   __device__ void operator()(uint32_t t, const ArenaState& s) const { store_tick(trace + (size_t)(t - 1) * slab_words, a, s); }
 };
 
-__global__ void __launch_bounds__(MAX_BLOCK) k_sim8(uint32_t* buf, const StaticData* __restrict__ w,
-                                                    const Controls* __restrict__ ctl, uint32_t* result, uint32_t n) {
+// One decision of every arena. With a variant array, arena a steps in worlds[variant[a]] (an index past the table leaves
+// the arena untouched and flags its result word); without one, every arena steps in worlds[0].
+template <bool TRACE>
+__global__ void __launch_bounds__(MAX_BLOCK) k_sim8(Sim8Args p) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
+  if (a >= p.n) return;
+  uint32_t v = p.variant ? p.variant[a] : 0u;
+  if (v >= p.nworlds) {
+    p.result[a] = SIM8_BAD_VARIANT;
+    return;
+  }
+  const StaticData& w = p.worlds[v];
   ArenaState s;
-  load_tick(s, buf, a);
+  load_tick(s, p.buf, a);
   Controls c[NUM_CARS];
-  for (int i = 0; i < NUM_CARS; ++i) c[i] = ctl[(size_t)a * NUM_CARS + i];
-  result[a] = run_decision(s, *w, c, TICKS_PER_DECISION, NoTrace{});
-  store_tick(buf, a, s);
+  for (int i = 0; i < NUM_CARS; ++i) c[i] = p.ctl[(size_t)a * NUM_CARS + i];
+  if (TRACE) p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, TraceOut{p.trace, aosoa_words(p.n), a});
+  else p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, NoTrace{});
+  store_tick(p.buf, a, s);
 }
-__global__ void __launch_bounds__(MAX_BLOCK) k_sim8_trace(uint32_t* buf, const StaticData* __restrict__ w,
-                                                          const Controls* __restrict__ ctl, uint32_t* result, uint32_t n,
-                                                          uint32_t* trace) {
+
+// Several decisions in one launch: each lane keeps its arena in its private copy from the first decision to the last
+// (one copy-in, one copy-out), with the controls of decision d at ctl[d * n * NUM_CARS ...] and its result words at
+// result[d * n ...]. Decision d + 1 starts where decision d ended (the goal state after a goal), as consecutive sim8
+// launches do; the mid-tick tail is cleared between decisions, as the tick-boundary copy clears it.
+__global__ void __launch_bounds__(MAX_BLOCK) k_run(Sim8Args p, uint32_t decisions) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
+  if (a >= p.n) return;
+  uint32_t v = p.variant ? p.variant[a] : 0u;
+  if (v >= p.nworlds) {
+    for (uint32_t d = 0; d < decisions; ++d) p.result[(size_t)d * p.n + a] = SIM8_BAD_VARIANT;
+    return;
+  }
+  const StaticData& w = p.worlds[v];
   ArenaState s;
-  load_tick(s, buf, a);
-  Controls c[NUM_CARS];
-  for (int i = 0; i < NUM_CARS; ++i) c[i] = ctl[(size_t)a * NUM_CARS + i];
-  result[a] = run_decision(s, *w, c, TICKS_PER_DECISION, TraceOut{trace, aosoa_words(n), a});
-  store_tick(buf, a, s);
+  load_tick(s, p.buf, a);
+  uint32_t* tail = reinterpret_cast<uint32_t*>(&s);
+  for (uint32_t d = 0; d < decisions; ++d) {
+    if (d > 0)
+      for (int i = TICK_WORDS; i < ARENA_WORDS; ++i) tail[i] = 0;
+    Controls c[NUM_CARS];
+    const Controls* cd = p.ctl + ((size_t)d * p.n + a) * NUM_CARS;
+    for (int i = 0; i < NUM_CARS; ++i) c[i] = cd[i];
+    p.result[(size_t)d * p.n + a] = run_decision(s, w, c, TICKS_PER_DECISION, NoTrace{});
+  }
+  store_tick(p.buf, a, s);
 }
 
 // A plain streaming copy of whole state buffers (16-byte vectors, grid-stride): the restore between timed runs, and the
@@ -136,11 +161,14 @@ hipError_t launch_step_tick(hipStream_t st, uint32_t* buf, const StaticData* w, 
   k_step_tick<<<grid_of(n, SIM8_BLOCK), SIM8_BLOCK, trav_lds_bytes(SIM8_BLOCK), st>>>(buf, w, n, status);
   return hipGetLastError();
 }
-hipError_t launch_sim8(hipStream_t st, uint32_t* buf, const StaticData* w, const Controls* ctl, uint32_t* result, uint32_t n,
-                       uint32_t block, uint32_t* trace) {
+hipError_t launch_run(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block) {
+  k_run<<<grid_of(p.n, block), block, trav_lds_bytes(block), st>>>(p, decisions);
+  return hipGetLastError();
+}
+hipError_t launch_sim8(hipStream_t st, const Sim8Args& p, uint32_t block) {
   const size_t lds = trav_lds_bytes(block);  // the BVH walks' traversal stacks
-  if (trace) k_sim8_trace<<<grid_of(n, block), block, lds, st>>>(buf, w, ctl, result, n, trace);
-  else k_sim8<<<grid_of(n, block), block, lds, st>>>(buf, w, ctl, result, n);
+  if (p.trace) k_sim8<true><<<grid_of(p.n, block), block, lds, st>>>(p);
+  else k_sim8<false><<<grid_of(p.n, block), block, lds, st>>>(p);
   return hipGetLastError();
 }
 

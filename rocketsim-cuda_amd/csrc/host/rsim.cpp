@@ -27,7 +27,8 @@ struct rsim_device {
 };
 struct rsim_world {
   rsim_device* dev;
-  StaticData* d;
+  StaticData* d;   // count StaticData back to back
+  uint32_t count;
 };
 struct rsim_states {
   rsim_device* dev;
@@ -188,23 +189,29 @@ int rsim_timer_end(rsim_device* dev, float* ms) {
 }
 
 // ---- world -------------------------------------------------------------------------------------------------------------------
-int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** out) {
+int rsim_upload_worlds(rsim_device* dev, const void* static_data, uint32_t count, rsim_world** out) {
   int rc = use(dev);
   if (rc) return rc;
-  if (!static_data || !out) return fail(RSIM_ERR_INVALID, "rsim_upload_world: null argument");
-  rsim_world* w = new (std::nothrow) rsim_world{dev, nullptr};
+  if (!static_data || !out || count == 0) return fail(RSIM_ERR_INVALID, "rsim_upload_worlds: null argument or no world");
+  if (count > RSIM_MAX_WORLDS) return fail(RSIM_ERR_INVALID, "rsim_upload_worlds: more than RSIM_MAX_WORLDS worlds");
+  rsim_world* w = new (std::nothrow) rsim_world{dev, nullptr, count};
   if (!w) return fail(RSIM_ERR_INVALID, "out of host memory");
-  hipError_t e = hipMalloc((void**)&w->d, sizeof(StaticData));
-  if (e == hipSuccess) e = hipMemcpyAsync(w->d, static_data, sizeof(StaticData), hipMemcpyHostToDevice, dev->stream);
+  const size_t bytes = sizeof(StaticData) * count;
+  hipError_t e = hipMalloc((void**)&w->d, bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(w->d, static_data, bytes, hipMemcpyHostToDevice, dev->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
   if (e != hipSuccess) {
     if (w->d) (void)hipFree(w->d);
     delete w;
-    return hip_fail(e, "rsim_upload_world");
+    return hip_fail(e, "rsim_upload_worlds");
   }
   *out = w;
   return RSIM_OK;
 }
+int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** out) {
+  return rsim_upload_worlds(dev, static_data, 1, out);
+}
+uint32_t rsim_world_count(const rsim_world* w) { return w ? w->count : 0; }
 int rsim_world_free(rsim_world* w) {
   if (!w) return RSIM_OK;
   int rc = use(w->dev);
@@ -367,20 +374,41 @@ uint64_t rsim_words_count(const rsim_words* b) { return b ? b->n : 0; }
 void* rsim_words_device_ptr(rsim_words* b) { return b ? b->d : nullptr; }
 
 // ---- kernels -----------------------------------------------------------------------------------------------------------------
-int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
-              uint32_t block, rsim_words* trace) {
+int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                       const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace) {
   int rc = use(dev);
   if (rc) return rc;
   if (!states || !world || !controls || !result) return fail(RSIM_ERR_INVALID, "rsim_sim8: null argument");
   if (block < 1 || block > RSIM_MAX_BLOCK) return fail(RSIM_ERR_INVALID, "rsim_sim8: block outside 1..RSIM_MAX_BLOCK");
   const uint32_t n = states->n;
   if (controls->n < (uint64_t)n * rs::NUM_CARS * 8 || result->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8: buffers too short");
+  if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8: variant buffer too short");
   if (trace && trace->n < (uint64_t)RSIM_TICKS_PER_DECISION * rs::dev::aosoa_words(n))
     return fail(RSIM_ERR_SHORT, "rsim_sim8: trace too short");
   if (!n) return RSIM_OK;
-  RSIM_HIP(rs::dev::launch_sim8(dev->stream, states->d, world->d, reinterpret_cast<const Controls*>(controls->d), result->d, n,
-                                block, trace ? trace->d : nullptr),
-           "sim8 launch");
+  rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
+                      reinterpret_cast<const Controls*>(controls->d), result->d, n, trace ? trace->d : nullptr};
+  RSIM_HIP(rs::dev::launch_sim8(dev->stream, p, block), "sim8 launch");
+  return RSIM_OK;
+}
+int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
+              uint32_t block, rsim_words* trace) {
+  return rsim_sim8_variants(dev, states, world, nullptr, controls, result, block, trace);
+}
+int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant, const rsim_words* controls,
+             rsim_words* results, uint32_t decisions, uint32_t block) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world || !controls || !results) return fail(RSIM_ERR_INVALID, "rsim_run: null argument");
+  if (block < 1 || block > RSIM_MAX_BLOCK) return fail(RSIM_ERR_INVALID, "rsim_run: block outside 1..RSIM_MAX_BLOCK");
+  const uint32_t n = states->n;
+  if (controls->n < (uint64_t)decisions * n * rs::NUM_CARS * 8 || results->n < (uint64_t)decisions * n)
+    return fail(RSIM_ERR_SHORT, "rsim_run: buffers too short");
+  if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_run: variant buffer too short");
+  if (!n || !decisions) return RSIM_OK;
+  rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
+                      reinterpret_cast<const Controls*>(controls->d), results->d, n, nullptr};
+  RSIM_HIP(rs::dev::launch_run(dev->stream, p, decisions, block), "run launch");
   return RSIM_OK;
 }
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {

@@ -99,6 +99,14 @@ class SimDevice {
     check(rsim_upload_world(d_, sd.data(), &w), "upload_static");
     return DeviceStatic(w);
   }
+  // A table of `worlds.size() / rsim_static_words()` StaticData for sim8_variants.
+  DeviceStatic upload_worlds(const std::vector<uint32_t>& worlds) {
+    const uint64_t sw = rsim_static_words();
+    if (worlds.empty() || worlds.size() % sw) throw Error(RSIM_ERR_INVALID, "upload_worlds: not whole StaticData");
+    rsim_world* w = nullptr;
+    check(rsim_upload_worlds(d_, worlds.data(), (uint32_t)(worlds.size() / sw), &w), "upload_worlds");
+    return DeviceStatic(w);
+  }
   // `states` holds n ArenaStates back to back (n * rsim_arena_words() words); packed to AoSoA-64 on upload.
   DeviceStates upload_states(const std::vector<uint32_t>& states) {
     const uint64_t aw = rsim_arena_words();
@@ -143,6 +151,13 @@ class SimDevice {
   void sim8(DeviceStates& states, const DeviceStatic& sd, const DeviceWords& controls, DeviceWords& result,
             uint32_t block = SIM8_BLOCK, DeviceWords* trace = nullptr) {
     check(rsim_sim8(d_, states.get(), sd.get(), controls.get(), result.get(), block, trace ? trace->get() : nullptr), "sim8");
+  }
+  // sim8 with arena a stepped in world variant[a] of a table (no counterpart in the reference).
+  void sim8_variants(DeviceStates& states, const DeviceStatic& worlds, const DeviceWords* variant, const DeviceWords& controls,
+                     DeviceWords& result, uint32_t block = SIM8_BLOCK, DeviceWords* trace = nullptr) {
+    check(rsim_sim8_variants(d_, states.get(), worlds.get(), variant ? variant->get() : nullptr, controls.get(), result.get(),
+                             block, trace ? trace->get() : nullptr),
+          "sim8_variants");
   }
   // The parity kernels (device.rs:95-141), synchronous; one PhaseStatus per arena.
   void identity(const DeviceStates& in, DeviceStates& out) { check(rsim_identity(d_, in.get(), out.get()), "identity"); }

@@ -25,6 +25,8 @@ TICKS_PER_DECISION = 8
 SIM8_SCORED_MASK = 0xFF
 SIM8_UNIMPLEMENTED = 0x80000000
 SIM8_OVERFLOW = 0x40000000
+SIM8_BAD_VARIANT = 0x20000000
+MAX_WORLDS = 1024
 SIM8_BLOCK = 64
 MAX_BLOCK = 256
 AOSOA_WIDTH = 64
@@ -58,6 +60,9 @@ def lib():
             "rsim_open": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int), "rsim_close": ([_vp], ctypes.c_int),
             "rsim_stream": ([_vp], _vp),
             "rsim_upload_world": ([_vp, _vp, ctypes.POINTER(_vp)], ctypes.c_int), "rsim_world_free": ([_vp], ctypes.c_int),
+            "rsim_upload_worlds": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int), "rsim_world_count": ([_vp], u32),
+            "rsim_sim8_variants": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, _vp], ctypes.c_int),
+            "rsim_run": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, u32], ctypes.c_int),
             "rsim_upload_states": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_alloc_states": ([_vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_write_states": ([_vp, _vp, _vp], ctypes.c_int), "rsim_copy_states": ([_vp, _vp, _vp], ctypes.c_int),
@@ -245,6 +250,13 @@ class SimDevice:
                "rsim_upload_world")
         return DeviceStatic(self, h)
 
+    def upload_worlds(self, worlds):
+        """A table of StaticData (uint32[count, STATIC_WORDS]) for rsim_sim8_variants."""
+        w = _words(worlds).reshape(-1, STATIC_WORDS)
+        h = _vp()
+        _check(lib().rsim_upload_worlds(self.handle, _ptr(w), w.shape[0], ctypes.byref(h)), "rsim_upload_worlds")
+        return DeviceStatic(self, h)
+
     def upload_states(self, states):
         s = _words(states).reshape(-1, ARENA_WORDS)
         h = _vp()
@@ -320,6 +332,17 @@ class SimDevice:
         """One decision of every arena, in place, asynchronously (device.rs:151-168)."""
         _check(lib().rsim_sim8(self.handle, states.handle, sd.handle, controls.handle, result.handle, block,
                                trace.handle if trace is not None else None), "rsim_sim8")
+
+    def sim8_variants(self, states, worlds, variant, controls, result, block=SIM8_BLOCK, trace=None):
+        """sim8 with arena a stepped in worlds[variant[a]] (variant: a DeviceWords of n indices, or None for world 0)."""
+        _check(lib().rsim_sim8_variants(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
+                                        controls.handle, result.handle, block, trace.handle if trace is not None else None),
+               "rsim_sim8_variants")
+
+    def run(self, states, worlds, controls, results, decisions, variant=None, block=SIM8_BLOCK):
+        """`decisions` decisions in one launch (controls: decisions x n x 16 words, results: decisions x n words)."""
+        _check(lib().rsim_run(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
+                              controls.handle, results.handle, decisions, block), "rsim_run")
 
     def sync(self):
         _check(lib().rsim_sync(self.handle), "rsim_sync")

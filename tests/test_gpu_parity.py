@@ -228,3 +228,93 @@ def test_argument_checks(rsim, world, device, dev_world):
         device.sim8(d, dev_world, c, device.alloc_words(n - 1))
     with pytest.raises(rsim.RsimError):
         device.sim8(d, dev_world, c, r, trace=device.alloc_words(8))
+
+
+def test_world_variants_seven_pairings(rsim, oracle, device):
+    """Open play over all seven preset pairings in one launch (a table of seven worlds, one index per arena), every
+    arena word-identical to the oracle run in its own world; an index past the table leaves the arena untouched."""
+    note("seven pairings in one launch, 4096 arenas x 3 decisions")
+    n = 4096
+    worlds = np.stack([rsim.build_world(p, p) for p in range(7)])
+    variant = np.random.default_rng(81).integers(0, 7, n).astype(np.uint32)
+    states = np.concatenate([rsim.sample_arenas(worlds[variant[a]], rsim.SCEN_OPEN, 82, 1, first=a) for a in range(n)])
+    dw = device.upload_worlds(worlds)
+    dv = device.upload_words(variant)
+    d = device.upload_states(states)
+    r = device.alloc_words(n)
+    cpu = states.copy()
+    for k in range(3):
+        ctl = rsim.sample_controls(83, k, n)
+        device.sim8_variants(d, dw, dv, device.upload_controls(ctl), r)
+        device.sync()
+        got_r = device.read_words(r)
+        for p in range(7):
+            idx = np.nonzero(variant == p)[0]
+            sub = np.ascontiguousarray(cpu[idx])
+            res = oracle.sim8(sub, worlds[p], np.ascontiguousarray(ctl[idx]), threads=8)
+            cpu[idx] = sub
+            assert np.array_equal(got_r[idx], res), "pairing %d decision %d" % (p, k)
+    got = device.download_states(d)
+    fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
+    assert fd is None, "arena %d word %d (pairing %d)" % (fd[0], fd[1], variant[fd[0]])
+    # an out-of-table index: untouched state, flagged result
+    bad = variant.copy()
+    bad[5] = 7
+    before = device.download_states(d)
+    device.sim8_variants(d, dw, device.upload_words(bad), device.upload_controls(rsim.sample_controls(84, 0, n)), r)
+    device.sync()
+    after = device.download_states(d)
+    assert device.read_words(r)[5] == rsim.SIM8_BAD_VARIANT
+    assert np.array_equal(after[5], before[5])
+
+
+def test_world_variants_mutators(rsim, oracle, device):
+    """Per-arena mutators (gravity, boost acceleration, ball radius) and presets from a table of 64 worlds, the scale
+    workload's arena mix (kickoff, open play, contact 1:1:1)."""
+    note("64 mutator worlds, 2048 arenas x 2 decisions")
+    rng = np.random.default_rng(91)
+    nw, n = 64, 2048
+    worlds = np.stack([rsim.build_world(int(rng.integers(0, 7)), int(rng.integers(0, 7)), float(rng.uniform(-800, -500)),
+                                        float(rng.uniform(0.8, 1.2)), float(rng.uniform(91.25, 100))) for _ in range(nw)])
+    variant = rng.integers(0, nw, n).astype(np.uint32)
+    states = np.concatenate([rsim.sample_arenas(worlds[variant[a]], rsim.SCEN_MIXED, 92, 1, first=a) for a in range(n)])
+    dw = device.upload_worlds(worlds)
+    dv = device.upload_words(variant)
+    d = device.upload_states(states)
+    r = device.alloc_words(n)
+    cpu = states.copy()
+    for k in range(2):
+        ctl = rsim.sample_controls(93, k, n)
+        device.sim8_variants(d, dw, dv, device.upload_controls(ctl), r)
+        device.sync()
+        got_r = device.read_words(r)
+        for p in np.unique(variant):
+            idx = np.nonzero(variant == p)[0]
+            sub = np.ascontiguousarray(cpu[idx])
+            res = oracle.sim8(sub, worlds[p], np.ascontiguousarray(ctl[idx]))
+            cpu[idx] = sub
+            assert np.array_equal(got_r[idx], res), "world %d decision %d" % (p, k)
+    got = device.download_states(d)
+    fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
+    assert fd is None, "arena %d word %d" % fd
+
+
+def test_run_equals_consecutive_sim8(rsim, oracle, world, device, dev_world):
+    """rsim_run over five decisions in one launch: the same states and result words as five sim8 launches, and as the
+    oracle."""
+    note("rsim_run 5 decisions, 1500 arenas of the 1:1:1 mix")
+    n, D = 1500, 5
+    states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 101, n)
+    ctl = [rsim.sample_controls(102, d, n) for d in range(D)]
+    d = device.upload_states(states)
+    c = device.upload_words(np.concatenate(ctl))
+    r = device.alloc_words(D * n)
+    device.run(d, dev_world, c, r, D)
+    device.sync()
+    got, got_r = device.download_states(d), device.read_words(r).reshape(D, n)
+    gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+    assert np.array_equal(got[:, :TICK_WORDS], gs[:, :TICK_WORDS])
+    for k in range(D):
+        assert np.array_equal(got_r[k], gr[k]), "decision %d" % k
+    cs, cr = cpu_decisions(oracle, world, states, ctl)
+    assert_same(got, list(got_r), cs, cr, "rsim_run")
