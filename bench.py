@@ -69,6 +69,49 @@ def cpu_baseline_leg(world, states0, controls, gpu_states, gpu_results, sample, 
             "seconds": dt}, diffs
 
 
+def cost_regrouped_legs(rsim, dev, world_host, dw, d_init, d_work, controls, K, n, final, results, block):
+    """The learned cost model's legs: labels from the simulator (a sample of arenas, one wave each), a few epochs of
+    training on the device, then the same K decisions with the arenas regrouped by predicted cost before every decision
+    ('sorted': cost-homogeneous waves; 'balanced': each wave dealt one arena of every cost rank). Each leg's states and
+    result words are compared with the plain run's."""
+    import rsim.cost as cm
+
+    sample = min(8192, n)
+    t0 = time.perf_counter()
+    # training arenas come from another seed than the timed batch
+    feats, targets, contacts = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, rsim.SCEN_CONTACT, 77, sample),
+                                              [rsim.sample_controls(78, d, sample) for d in range(2)], decisions=2)
+    model = cm.CostModel(dev, seed=1)
+    curve = model.train(feats, targets, contacts, epochs=20, batch=2048, lr=3e-3)
+    prep = time.perf_counter() - t0
+    pred = model.predict(dev.upload_words(feats.view(np.uint32).ravel()), feats.shape[0])
+    dev.sync()
+    p = dev.read_words(pred, 2 * feats.shape[0]).view(np.float32).reshape(-1, 2)[:, 0]
+    corr = float(np.corrcoef(p, targets)[0, 1])
+    out = {"training": {"samples": int(feats.shape[0]), "epochs": 20, "loss_first": curve[0], "loss_last": curve[-1],
+                        "pearson_pred_vs_measured_log_cost": corr, "seconds": prep}}
+    d_ctl = [dev.upload_controls(c) for c in controls[:K]]
+    res = dev.alloc_words(n)
+    for mode in ("sorted", "balanced"):
+        dev.copy_states(d_work, d_init)
+        dev.sync()
+        g = cm.Regrouper(dev, model, dw, d_work, mode, block)
+        t1 = time.perf_counter()
+        for k in range(K):
+            g.regroup()
+            g.step(d_ctl[k], res)
+            if k == K - 1:
+                dev.sync()
+        wall = time.perf_counter() - t1
+        same = bool(np.array_equal(dev.read_words(res), results[-1]))
+        fin = dev.download_states(g.states_in_arena_order())
+        same = same and bool(np.array_equal(fin[:, :2567], final[:, :2567]))
+        stepped = sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
+        out[mode] = {"arena_ticks_per_s": stepped / wall, "ms_per_decision": wall * 1000.0 / K, "identical_to_plain": same,
+                     "includes": "features, MLP inference, host argsort, state gather, sim8, result scatter per decision"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +121,7 @@ def main():
     ap.add_argument("--arenas", type=int, default=ARENAS)
     ap.add_argument("--cpu-sample", type=int, default=ARENAS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cost", action="store_true", help="skip the learned cost-model legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this stepper runs on one GPU; --gpus %d is not supported (arenas are independent, so N GPUs would "
@@ -145,6 +189,11 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    cost = None
+    if not args.no_cost:
+        print("[bench] cost model: labels, training, regrouped decisions", file=sys.stderr, flush=True)
+        cost = cost_regrouped_legs(rsim, dev, world, dw, d_init, d_work, controls, K, n, final, results, args.block)
+
     cpu, diffs = None, None
     if not args.no_cpu:
         print("[bench] CPU baseline on %d arenas" % min(args.cpu_sample, n), file=sys.stderr, flush=True)
@@ -176,6 +225,7 @@ def main():
         "arena_ticks_nominal": nominal,
         "nominal_arena_ticks_per_s": nominal / wall,
         "flagged_arenas": flags,
+        "cost_regrouped": cost,
         "multi_decision_launch": {"arena_ticks_per_s": stepped / run_wall, "kernel_ms": run_ms, "kernel": "k_run",
                                   "identical_to_plain": run_same},
         "word_diffs": diffs,

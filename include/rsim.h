@@ -113,6 +113,11 @@ int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, co
  * counterpart in the reference; with a one-world table and a null variant it is rsim_sim8. */
 int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
                        const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace);
+/* rsim_sim8_variants that also writes each lane's wall-clock ticks (100 MHz) for its decision into `cycles` (>= 2 * n
+ * words: n u64). The lanes of a wave share its clock; launched with block 1 every arena has a wave of its own and the
+ * count is that arena's own cost (the cost model's labels). No counterpart in the reference. */
+int rsim_sim8_timed(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace, rsim_words* cycles);
 
 /* `decisions` consecutive decisions in one launch, asynchronous: the same results as that many rsim_sim8_variants calls
  * (decision d + 1 starts from decision d's written-back state), with each arena held on chip from the first decision to
@@ -121,6 +126,28 @@ int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* 
  * whose controls are known ahead (replays, scripted opponents, benchmarks). */
 int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant, const rsim_words* controls,
              rsim_words* results, uint32_t decisions, uint32_t block);
+
+/* ---- the learned cost model and arena regrouping (no counterpart in the reference) ----------------------------------------
+ * A 48 -> 64 -> 64 -> 2 MLP over per-arena features predicts (log of the arena's decision cost, contact logit). Inference
+ * runs on bf16 MFMA with fp32 accumulation; training is the library's own forward/backward/Adam kernels. Parameters: a
+ * u32 buffer of RSIM_COST_PARAMS fp32 words: W1[48][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]. All asynchronous. */
+#define RSIM_COST_FEATURES 48
+#define RSIM_COST_PARAMS 7426
+#define RSIM_COST_ACT_WORDS 306 /* training scratch words per batch row */
+int rsim_cost_features(rsim_device* dev, const rsim_states* states, rsim_words* features); /* >= n * 48 fp32 */
+int rsim_cost_predict(rsim_device* dev, const rsim_words* params, const rsim_words* features, uint32_t n, rsim_words* out); /* out: n x 2 */
+/* One batch: loss[i] per row (squared error on target + logistic loss on contact) and grad = d(mean loss)/d(params).
+ * target, contact: batch fp32; scratch: batch * RSIM_COST_ACT_WORDS words. */
+int rsim_cost_gradients(rsim_device* dev, const rsim_words* params, const rsim_words* features, const rsim_words* target,
+                        const rsim_words* contact, uint32_t batch, rsim_words* scratch, rsim_words* loss, rsim_words* grad);
+int rsim_cost_adam(rsim_device* dev, rsim_words* params, rsim_words* m, rsim_words* v, const rsim_words* grad, float lr, uint32_t step);
+/* Uploads a host permutation of 0..n-1 (checked) for the regrouping calls. */
+int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsim_words* dst);
+/* dst arena i = src arena perm[i] (whole AoSoA-64 states; src and dst distinct, same n). */
+int rsim_gather_states(rsim_device* dev, const rsim_states* src, const rsim_words* perm, rsim_states* dst);
+/* Items of words_per_item words: gather (dst[i] = src[perm[i]]) or, with scatter != 0, dst[perm[i]] = src[i]. */
+int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words* perm, uint32_t n, uint32_t words_per_item,
+                       int scatter, rsim_words* dst);
 
 /* The debug and parity kernels (device.rs:90-141), synchronous. status_out: n ints (PhaseStatus per arena) or null. */
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out);      /* copy-in, copy-out */

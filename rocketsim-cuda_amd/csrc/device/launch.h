@@ -32,6 +32,7 @@ struct Sim8Args {
   uint32_t* result;          // n result words
   uint32_t n;
   uint32_t* trace;           // null, or TICKS_PER_DECISION slabs of aosoa_words(n)
+  uint64_t* cycles;          // null, or n: the lane's wall-clock ticks (100 MHz) for its decision
 };
 hipError_t launch_sim8(hipStream_t st, const Sim8Args& p, uint32_t block);
 hipError_t launch_run(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block);  // ctl/result: per decision

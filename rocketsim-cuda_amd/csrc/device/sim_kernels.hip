@@ -84,16 +84,20 @@ struct TraceOut {  // the tick-boundary words after tick t into slab t - 1 of th
 
 // One decision of every arena. With a variant array, arena a steps in worlds[variant[a]] (an index past the table leaves
 // the arena untouched and flags its result word); without one, every arena steps in worlds[0].
-template <bool TRACE>
+template <bool TRACE, bool VARIANTS>
 __global__ void __launch_bounds__(MAX_BLOCK) k_sim8(Sim8Args p) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= p.n) return;
-  uint32_t v = p.variant ? p.variant[a] : 0u;
-  if (v >= p.nworlds) {
-    p.result[a] = SIM8_BAD_VARIANT;
-    return;
+  uint32_t v = 0;
+  if (VARIANTS) {
+    v = p.variant[a];
+    if (v >= p.nworlds) {
+      p.result[a] = SIM8_BAD_VARIANT;
+      return;
+    }
   }
-  const StaticData& w = p.worlds[v];
+  const StaticData& w = p.worlds[v];  // without variants the same world for every lane: a uniform pointer
+  const uint64_t t0 = p.cycles ? wall_clock64() : 0;
   ArenaState s;
   load_tick(s, p.buf, a);
   Controls c[NUM_CARS];
@@ -101,6 +105,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_sim8(Sim8Args p) {
   if (TRACE) p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, TraceOut{p.trace, aosoa_words(p.n), a});
   else p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, NoTrace{});
   store_tick(p.buf, a, s);
+  if (p.cycles) p.cycles[a] = wall_clock64() - t0;  // a wave's lanes share its clock: with one lane per work-group
+                                                     // (block 1) this is the arena's own cost
 }
 
 // Several decisions in one launch: each lane keeps its arena in its private copy from the first decision to the last
@@ -167,8 +173,11 @@ hipError_t launch_run(hipStream_t st, const Sim8Args& p, uint32_t decisions, uin
 }
 hipError_t launch_sim8(hipStream_t st, const Sim8Args& p, uint32_t block) {
   const size_t lds = trav_lds_bytes(block);  // the BVH walks' traversal stacks
-  if (p.trace) k_sim8<true><<<grid_of(p.n, block), block, lds, st>>>(p);
-  else k_sim8<false><<<grid_of(p.n, block), block, lds, st>>>(p);
+  const dim3 g = grid_of(p.n, block);
+  if (p.trace && p.variant) k_sim8<true, true><<<g, block, lds, st>>>(p);
+  else if (p.trace) k_sim8<true, false><<<g, block, lds, st>>>(p);
+  else if (p.variant) k_sim8<false, true><<<g, block, lds, st>>>(p);
+  else k_sim8<false, false><<<g, block, lds, st>>>(p);
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "device/cost_model.h"
 #include "device/launch.h"
 
 using rs::ArenaState;
@@ -374,8 +375,8 @@ uint64_t rsim_words_count(const rsim_words* b) { return b ? b->n : 0; }
 void* rsim_words_device_ptr(rsim_words* b) { return b ? b->d : nullptr; }
 
 // ---- kernels -----------------------------------------------------------------------------------------------------------------
-int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
-                       const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace) {
+int rsim_sim8_timed(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace, rsim_words* cycles) {
   int rc = use(dev);
   if (rc) return rc;
   if (!states || !world || !controls || !result) return fail(RSIM_ERR_INVALID, "rsim_sim8: null argument");
@@ -383,17 +384,23 @@ int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* 
   const uint32_t n = states->n;
   if (controls->n < (uint64_t)n * rs::NUM_CARS * 8 || result->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8: buffers too short");
   if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8: variant buffer too short");
+  if (cycles && cycles->n < 2ull * n) return fail(RSIM_ERR_SHORT, "rsim_sim8: cycles buffer too short");
   if (trace && trace->n < (uint64_t)RSIM_TICKS_PER_DECISION * rs::dev::aosoa_words(n))
     return fail(RSIM_ERR_SHORT, "rsim_sim8: trace too short");
   if (!n) return RSIM_OK;
   rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
-                      reinterpret_cast<const Controls*>(controls->d), result->d, n, trace ? trace->d : nullptr};
+                      reinterpret_cast<const Controls*>(controls->d), result->d, n, trace ? trace->d : nullptr, nullptr};
+  if (cycles) p.cycles = reinterpret_cast<uint64_t*>(cycles->d);
   RSIM_HIP(rs::dev::launch_sim8(dev->stream, p, block), "sim8 launch");
   return RSIM_OK;
 }
+int rsim_sim8_variants(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                       const rsim_words* controls, rsim_words* result, uint32_t block, rsim_words* trace) {
+  return rsim_sim8_timed(dev, states, world, variant, controls, result, block, trace, nullptr);
+}
 int rsim_sim8(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* controls, rsim_words* result,
               uint32_t block, rsim_words* trace) {
-  return rsim_sim8_variants(dev, states, world, nullptr, controls, result, block, trace);
+  return rsim_sim8_timed(dev, states, world, nullptr, controls, result, block, trace, nullptr);
 }
 int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant, const rsim_words* controls,
              rsim_words* results, uint32_t decisions, uint32_t block) {
@@ -407,8 +414,93 @@ int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, con
   if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_run: variant buffer too short");
   if (!n || !decisions) return RSIM_OK;
   rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
-                      reinterpret_cast<const Controls*>(controls->d), results->d, n, nullptr};
+                      reinterpret_cast<const Controls*>(controls->d), results->d, n, nullptr, nullptr};
   RSIM_HIP(rs::dev::launch_run(dev->stream, p, decisions, block), "run launch");
+  return RSIM_OK;
+}
+// ---- the cost model and regrouping -------------------------------------------------------------------------------------------
+int rsim_cost_features(rsim_device* dev, const rsim_states* states, rsim_words* features) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !features) return fail(RSIM_ERR_INVALID, "rsim_cost_features: null argument");
+  if (features->n < (uint64_t)states->n * RSIM_COST_FEATURES) return fail(RSIM_ERR_SHORT, "rsim_cost_features: buffer too short");
+  if (!states->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_features(dev->stream, states->d, states->n, reinterpret_cast<float*>(features->d)), "features");
+  return RSIM_OK;
+}
+int rsim_cost_predict(rsim_device* dev, const rsim_words* params, const rsim_words* features, uint32_t n, rsim_words* out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!params || !features || !out) return fail(RSIM_ERR_INVALID, "rsim_cost_predict: null argument");
+  if (params->n < RSIM_COST_PARAMS || features->n < (uint64_t)n * RSIM_COST_FEATURES || out->n < 2ull * n)
+    return fail(RSIM_ERR_SHORT, "rsim_cost_predict: buffers too short");
+  if (!n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_mlp_infer(dev->stream, reinterpret_cast<const float*>(params->d), reinterpret_cast<const float*>(features->d),
+                                     n, reinterpret_cast<float*>(out->d)),
+           "cost inference");
+  return RSIM_OK;
+}
+int rsim_cost_gradients(rsim_device* dev, const rsim_words* params, const rsim_words* features, const rsim_words* target,
+                        const rsim_words* contact, uint32_t batch, rsim_words* scratch, rsim_words* loss, rsim_words* grad) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!params || !features || !target || !contact || !scratch || !loss || !grad) return fail(RSIM_ERR_INVALID, "rsim_cost_gradients: null argument");
+  if (params->n < RSIM_COST_PARAMS || grad->n < RSIM_COST_PARAMS || features->n < (uint64_t)batch * RSIM_COST_FEATURES ||
+      target->n < batch || contact->n < batch || loss->n < batch || scratch->n < (uint64_t)batch * rs::dev::COST_ACT_WORDS)
+    return fail(RSIM_ERR_SHORT, "rsim_cost_gradients: buffers too short");
+  if (!batch) return fail(RSIM_ERR_INVALID, "rsim_cost_gradients: empty batch");
+  RSIM_HIP(rs::dev::launch_mlp_train(dev->stream, reinterpret_cast<const float*>(params->d), reinterpret_cast<const float*>(features->d),
+                                     reinterpret_cast<const float*>(target->d), reinterpret_cast<const float*>(contact->d), batch,
+                                     reinterpret_cast<float*>(scratch->d), reinterpret_cast<float*>(loss->d),
+                                     reinterpret_cast<float*>(grad->d)),
+           "cost gradients");
+  return RSIM_OK;
+}
+int rsim_cost_adam(rsim_device* dev, rsim_words* params, rsim_words* m, rsim_words* v, const rsim_words* grad, float lr, uint32_t step) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!params || !m || !v || !grad || step == 0) return fail(RSIM_ERR_INVALID, "rsim_cost_adam: null argument or step 0");
+  if (params->n < RSIM_COST_PARAMS || m->n < RSIM_COST_PARAMS || v->n < RSIM_COST_PARAMS || grad->n < RSIM_COST_PARAMS)
+    return fail(RSIM_ERR_SHORT, "rsim_cost_adam: buffers too short");
+  float bc1 = 1.0f - powf(0.9f, (float)step), bc2 = 1.0f - powf(0.999f, (float)step);
+  RSIM_HIP(rs::dev::launch_adam(dev->stream, reinterpret_cast<float*>(params->d), reinterpret_cast<float*>(m->d),
+                                reinterpret_cast<float*>(v->d), reinterpret_cast<const float*>(grad->d), lr, bc1, bc2),
+           "adam");
+  return RSIM_OK;
+}
+namespace {
+bool is_permutation(const uint32_t* perm, uint32_t n) {
+  std::vector<uint8_t> seen(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (perm[i] >= n || seen[perm[i]]) return false;
+    seen[perm[i]] = 1;
+  }
+  return true;
+}
+}  // namespace
+int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsim_words* dst) {
+  if (!perm && n) return fail(RSIM_ERR_INVALID, "rsim_set_permutation: null permutation");
+  if (!is_permutation(perm, n)) return fail(RSIM_ERR_INVALID, "rsim_set_permutation: not a permutation of 0..n-1");
+  return rsim_write_words(dev, dst, perm, n);
+}
+int rsim_gather_states(rsim_device* dev, const rsim_states* src, const rsim_words* perm, rsim_states* dst) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!src || !perm || !dst || src->n != dst->n || src == dst) return fail(RSIM_ERR_INVALID, "rsim_gather_states: bad buffers");
+  if (perm->n < src->n) return fail(RSIM_ERR_SHORT, "rsim_gather_states: permutation too short");
+  if (!src->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_gather_states(dev->stream, src->d, perm->d, dst->d, src->n), "gather states");
+  return RSIM_OK;
+}
+int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words* perm, uint32_t n, uint32_t words_per_item,
+                       int scatter, rsim_words* dst) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!src || !perm || !dst || src == dst) return fail(RSIM_ERR_INVALID, "rsim_permute_words: bad buffers");
+  if (perm->n < n || src->n < (uint64_t)n * words_per_item || dst->n < (uint64_t)n * words_per_item)
+    return fail(RSIM_ERR_SHORT, "rsim_permute_words: buffers too short");
+  if (!n || !words_per_item) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_gather_words(dev->stream, src->d, perm->d, dst->d, n, words_per_item, scatter != 0), "permute words");
   return RSIM_OK;
 }
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {

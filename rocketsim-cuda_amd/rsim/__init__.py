@@ -63,6 +63,14 @@ def lib():
             "rsim_upload_worlds": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int), "rsim_world_count": ([_vp], u32),
             "rsim_sim8_variants": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, _vp], ctypes.c_int),
             "rsim_run": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, u32], ctypes.c_int),
+            "rsim_sim8_timed": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, _vp, _vp], ctypes.c_int),
+            "rsim_cost_features": ([_vp, _vp, _vp], ctypes.c_int),
+            "rsim_cost_predict": ([_vp, _vp, _vp, u32, _vp], ctypes.c_int),
+            "rsim_cost_gradients": ([_vp, _vp, _vp, _vp, _vp, u32, _vp, _vp, _vp], ctypes.c_int),
+            "rsim_cost_adam": ([_vp, _vp, _vp, _vp, _vp, f32, u32], ctypes.c_int),
+            "rsim_set_permutation": ([_vp, _vp, u32, _vp], ctypes.c_int),
+            "rsim_gather_states": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+            "rsim_permute_words": ([_vp, _vp, _vp, u32, u32, ctypes.c_int, _vp], ctypes.c_int),
             "rsim_upload_states": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_alloc_states": ([_vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_write_states": ([_vp, _vp, _vp], ctypes.c_int), "rsim_copy_states": ([_vp, _vp, _vp], ctypes.c_int),

@@ -28,7 +28,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HIP_SOURCES = [os.path.join(CSRC, "device", "sim_kernels.hip"), os.path.join(CSRC, "host", "rsim.cpp")]
+HIP_SOURCES = [os.path.join(CSRC, "device", "sim_kernels.hip"), os.path.join(CSRC, "device", "cost_model.hip"),
+               os.path.join(CSRC, "host", "rsim.cpp")]
 HOST_SOURCES = [os.path.join(CSRC, "host", "world.cpp")]
 
 
