@@ -69,46 +69,90 @@ def cpu_baseline_leg(world, states0, controls, gpu_states, gpu_results, sample, 
             "seconds": dt}, diffs
 
 
-def cost_regrouped_legs(rsim, dev, world_host, dw, d_init, d_work, controls, K, n, final, results, block):
-    """The learned cost model's legs: labels from the simulator (a sample of arenas, one wave each), a few epochs of
-    training on the device, then the same K decisions with the arenas regrouped by predicted cost before every decision
-    ('sorted': cost-homogeneous waves; 'balanced': each wave dealt one arena of every cost rank). Each leg's states and
-    result words are compared with the plain run's."""
+def stepped_ticks(results):
+    return sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
+
+
+def train_cost_model(rsim, dev, world_host, dw):
+    """The cost model trained on labels the simulator emits: 8,192 arenas of each of three scenario mixes (seeds apart
+    from every timed batch) stepped 6 decisions with one arena per wave and each lane's clock recorded."""
     import rsim.cost as cm
 
-    sample = min(8192, n)
     t0 = time.perf_counter()
-    # training arenas come from another seed than the timed batch
-    feats, targets, contacts = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, rsim.SCEN_CONTACT, 77, sample),
-                                              [rsim.sample_controls(78, d, sample) for d in range(2)], decisions=2)
+    F, T, C = [], [], []
+    for scen, seed in ((rsim.SCEN_CONTACT, 77), (rsim.SCEN_OPEN_CONTACT, 79), (rsim.SCEN_MIXED, 81)):
+        f, t, c = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, scen, seed, 8192),
+                                 [rsim.sample_controls(seed + 1, d, 8192) for d in range(6)], decisions=6)
+        F.append(f), T.append(t), C.append(c)
+    feats, targets, contacts = np.concatenate(F), np.concatenate(T), np.concatenate(C)
+    label_s = time.perf_counter() - t0
     model = cm.CostModel(dev, seed=1)
-    curve = model.train(feats, targets, contacts, epochs=20, batch=2048, lr=3e-3)
-    prep = time.perf_counter() - t0
-    pred = model.predict(dev.upload_words(feats.view(np.uint32).ravel()), feats.shape[0])
-    dev.sync()
-    p = dev.read_words(pred, 2 * feats.shape[0]).view(np.float32).reshape(-1, 2)[:, 0]
-    corr = float(np.corrcoef(p, targets)[0, 1])
-    out = {"training": {"samples": int(feats.shape[0]), "epochs": 20, "loss_first": curve[0], "loss_last": curve[-1],
-                        "pearson_pred_vs_measured_log_cost": corr, "seconds": prep}}
-    d_ctl = [dev.upload_controls(c) for c in controls[:K]]
-    res = dev.alloc_words(n)
-    for mode in ("sorted", "balanced"):
+    curve = model.train(feats, targets, contacts, epochs=15, batch=4096, lr=3e-3)
+    # quality on held-out arenas: Pearson correlation of predicted and measured log cost, trained vs untrained
+    f, t, _ = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, rsim.SCEN_OPEN_CONTACT, 91, 4096),
+                             [rsim.sample_controls(92, d, 4096) for d in range(3)], decisions=3)
+    d_f = dev.upload_words(f.view(np.uint32).ravel())
+
+    def corr(m):
+        out = m.predict(d_f, f.shape[0])
+        dev.sync()
+        return float(np.corrcoef(dev.read_words(out, 2 * f.shape[0]).view(np.float32).reshape(-1, 2)[:, 0], t)[0, 1])
+
+    info = {"samples": int(feats.shape[0]), "epochs": 15, "loss_first": curve[0], "loss_last": curve[-1],
+            "label_seconds": label_s, "seconds": time.perf_counter() - t0,
+            "heldout_pearson_log_cost": {"trained": corr(model), "untrained": corr(cm.CostModel(dev, seed=1))}}
+    return model, info
+
+
+def compare_orders(rsim, dev, dw, model, states_host, controls, block, plain=None):
+    """K decisions of one batch in the plain order (per-launch sim8) and regrouped by predicted cost before every decision
+    ('sorted': cost-homogeneous waves, most expensive first; 'balanced': each wave dealt one arena of every cost rank).
+    Device time over all K decisions (regrouping included), and whether every result word and final state equals the
+    plain run's."""
+    import rsim.cost as cm
+
+    n, K = states_host.shape[0], len(controls)
+    d_init = dev.upload_states(states_host)
+    d_work = dev.alloc_states(n)
+    d_ctl = [dev.upload_controls(c) for c in controls]
+    d_res = [dev.alloc_words(n) for _ in range(K)]
+    out = {"arenas": n, "decisions": K}
+    if plain is None:
         dev.copy_states(d_work, d_init)
         dev.sync()
+        dev.timer_begin()
+        for k in range(K):
+            dev.sim8(d_work, dw, d_ctl[k], d_res[k], block)
+        ms = dev.timer_end()
+        results = [dev.read_words(r) for r in d_res]
+        final = dev.download_states(d_work)
+        stepped = stepped_ticks(results)
+        out["plain"] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K}
+    else:
+        results, final, ms = plain
+        stepped = stepped_ticks(results)
+        out["plain"] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K, "note": "the timed run"}
+    for mode in ("sorted", "balanced"):
+        dev.copy_states(d_work, d_init)
         g = cm.Regrouper(dev, model, dw, d_work, mode, block)
-        t1 = time.perf_counter()
+        dev.sync()
+        dev.timer_begin()
         for k in range(K):
             g.regroup()
-            g.step(d_ctl[k], res)
-            if k == K - 1:
-                dev.sync()
-        wall = time.perf_counter() - t1
-        same = bool(np.array_equal(dev.read_words(res), results[-1]))
+            g.step(d_ctl[k], d_res[k])
+        ms = dev.timer_end()
+        same = all(np.array_equal(dev.read_words(d_res[k]), results[k]) for k in range(K))
         fin = dev.download_states(g.states_in_arena_order())
         same = same and bool(np.array_equal(fin[:, :2567], final[:, :2567]))
-        stepped = sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
-        out[mode] = {"arena_ticks_per_s": stepped / wall, "ms_per_decision": wall * 1000.0 / K, "identical_to_plain": same,
-                     "includes": "features, MLP inference, host argsort, state gather, sim8, result scatter per decision"}
+        dev.sync()
+        dev.timer_begin()
+        for k in range(K):
+            g.regroup()
+        reg_ms = dev.timer_end()
+        out[mode] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K,
+                     "regroup_ms_per_decision": reg_ms / K, "identical_to_plain": same,
+                     "vs_plain": out["plain"]["ms_per_decision"] / (ms / K)}
+        del g
     return out
 
 
@@ -189,10 +233,40 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    # the same K decisions replayed from a HIP graph, checked against the plain run
+    print("[bench] the same decisions from a HIP graph", file=sys.stderr, flush=True)
+    dev.copy_states(d_work, d_init)
+    dev.sync()
+    dev.capture_begin()
+    for k in range(K):
+        dev.sim8(d_work, dw, d_ctl[k], d_res[k], args.block)
+    graph = dev.capture_end()
+    dev.copy_states(d_work, d_init)
+    dev.sync()
+    dev.timer_begin()
+    graph.launch()
+    graph_ms = dev.timer_end()
+    graph_same = bool(np.array_equal(dev.download_states(d_work)[:, :2567], final[:, :2567]) and
+                      all(np.array_equal(dev.read_words(d_res[k]), results[k]) for k in range(K)))
+    graph.free()
+
     cost = None
     if not args.no_cost:
-        print("[bench] cost model: labels, training, regrouped decisions", file=sys.stderr, flush=True)
-        cost = cost_regrouped_legs(rsim, dev, world, dw, d_init, d_work, controls, K, n, final, results, args.block)
+        print("[bench] cost model: labels and training", file=sys.stderr, flush=True)
+        model, training = train_cost_model(rsim, dev, world, dw)
+        cost = {"training": training}
+        print("[bench] cost-regrouped decisions, this batch", file=sys.stderr, flush=True)
+        cost["contact65536"] = compare_orders(rsim, dev, dw, model, states0, controls[:K], args.block,
+                                              plain=(results, final, gpu_ms))
+        k2 = min(K, 20)
+        print("[bench] cost-regrouped decisions, 65,536 arenas 1:1 open play and contact", file=sys.stderr, flush=True)
+        cost["open_contact65536"] = compare_orders(
+            rsim, dev, dw, model, rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 2026, n),
+            [rsim.sample_controls(4343, d, n) for d in range(k2)], args.block)
+        print("[bench] cost-regrouped decisions, 131,072 arenas 1:1:1 kickoff, open play, contact", file=sys.stderr, flush=True)
+        cost["mixed131072"] = compare_orders(
+            rsim, dev, dw, model, rsim.sample_arenas(world, rsim.SCEN_MIXED, 2027, 2 * n),
+            [rsim.sample_controls(4444, d, 2 * n) for d in range(k2)], args.block)
 
     cpu, diffs = None, None
     if not args.no_cpu:
@@ -228,6 +302,7 @@ def main():
         "cost_regrouped": cost,
         "multi_decision_launch": {"arena_ticks_per_s": stepped / run_wall, "kernel_ms": run_ms, "kernel": "k_run",
                                   "identical_to_plain": run_same},
+        "graph_replay": {"arena_ticks_per_s": stepped / (graph_ms / 1000.0), "ms": graph_ms, "identical_to_plain": graph_same},
         "word_diffs": diffs,
     }
     print(json.dumps(line), flush=True)

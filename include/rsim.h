@@ -141,6 +141,18 @@ int rsim_cost_predict(rsim_device* dev, const rsim_words* params, const rsim_wor
 int rsim_cost_gradients(rsim_device* dev, const rsim_words* params, const rsim_words* features, const rsim_words* target,
                         const rsim_words* contact, uint32_t batch, rsim_words* scratch, rsim_words* loss, rsim_words* grad);
 int rsim_cost_adam(rsim_device* dev, rsim_words* params, rsim_words* m, rsim_words* v, const rsim_words* grad, float lr, uint32_t step);
+/* The order of n arenas by predicted cost, computed on the device with no host synchronisation: perm[i] = the current
+ * slot that moves to slot i. pred: n x 2 fp32 from rsim_cost_predict (column 0 read). The predicted normalised log cost
+ * is bucketed (RSIM_COST_ORDER_BUCKETS buckets of 1/64 below 3.0, above 3.0 in the first, NaN in the last) and the slots
+ * are sorted by bucket, most expensive first, stably (ties keep slot order), so the result is deterministic.
+ * RSIM_ORDER_SORTED: that order (cost-homogeneous waves of 64). RSIM_ORDER_BALANCED: the sorted ranks dealt round-robin
+ * over the ceil(n / 64) waves (every wave gets one arena of each cost rank; the last, partial wave only in the first
+ * n - 64 (waves - 1) rounds). work: rsim_cost_order_work_words(n) words of scratch. */
+#define RSIM_COST_ORDER_BUCKETS 256
+#define RSIM_ORDER_SORTED 0
+#define RSIM_ORDER_BALANCED 1
+uint64_t rsim_cost_order_work_words(uint32_t n);
+int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* work, rsim_words* perm);
 /* Uploads a host permutation of 0..n-1 (checked) for the regrouping calls. */
 int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsim_words* dst);
 /* dst arena i = src arena perm[i] (whole AoSoA-64 states; src and dst distinct, same n). */
@@ -154,6 +166,16 @@ int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out);   
 int rsim_identity_tick(rsim_device* dev, const rsim_states* in, rsim_states* out); /* tick-boundary copy: tail zeroed */
 int rsim_phase(rsim_device* dev, int k, rsim_states* states, const rsim_world* world, int32_t* status_out);
 int rsim_step_tick(rsim_device* dev, rsim_states* states, const rsim_world* world, int32_t* status_out);
+
+/* HIP graphs over the device's stream: between rsim_capture_begin and rsim_capture_end the asynchronous entry points
+ * (rsim_sim8*, rsim_run, rsim_copy_states, rsim_cost_features / predict / order, rsim_gather_states, rsim_permute_words)
+ * are recorded instead of run; a synchronising call fails inside a capture and ends it. rsim_graph_launch replays the
+ * recorded work on the stream (asynchronously) with the buffers it was recorded with. No counterpart in the reference. */
+typedef struct rsim_graph rsim_graph;
+int rsim_capture_begin(rsim_device* dev);
+int rsim_capture_end(rsim_device* dev, rsim_graph** out);
+int rsim_graph_launch(rsim_device* dev, const rsim_graph* graph);
+int rsim_graph_free(rsim_graph* graph);
 
 /* Timing on the device's stream: HIP events recorded before and after the work queued between begin and end. */
 int rsim_timer_begin(rsim_device* dev);
@@ -172,6 +194,7 @@ const char* rsim_last_error(void);
 #define RSIM_SCEN_GOALFRAME 4 /* the ball within 150 uu of a post or the crossbar, moving at it */
 #define RSIM_SCEN_CONTACT 5   /* 40% pinch, 30% head-on, 30% goal frame */
 #define RSIM_SCEN_MIXED 6     /* 1:1:1 kickoff, open play, contact */
+#define RSIM_SCEN_OPEN_CONTACT 7 /* 1:1 open play, contact */
 
 /* The StaticData of one world: the procedural Soccar arena, car 0 with preset0 and car 1 with preset1, and the mutators
  * (Soccar defaults: gravity_z -650, boost_accel_scale 1, ball_radius 91.25). */

@@ -11,6 +11,14 @@ constexpr int COST_W1 = 0, COST_B1 = 48 * 64, COST_W2 = COST_B1 + 64, COST_B2 = 
               COST_B3 = COST_W3 + 64 * 2, COST_PARAMS = COST_B3 + 2;
 constexpr int COST_ACT_WORDS = 48 + 64 + 64 + 2 + 64 + 64;  // per training arena: x, h1, h2, dy, dh1, dh2
 
+// The order by predicted cost: ORDER_BUCKETS buckets of the normalised log cost over [ORDER_HI - ORDER_BUCKETS /
+// ORDER_SCALE, ORDER_HI), bucket 0 the most expensive (above the range), NaN in the last.
+constexpr uint32_t ORDER_BUCKETS = 256, ORDER_BLOCK = 256, ORDER_SCAN = 1024;
+constexpr float ORDER_HI = 3.0f, ORDER_SCALE = 64.0f;
+constexpr int ORDER_SORTED = 0, ORDER_BALANCED = 1;
+uint64_t order_work_words(uint32_t n);
+hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int mode, uint32_t* work, uint32_t* perm);
+
 hipError_t launch_features(hipStream_t st, const uint32_t* states, uint32_t n, float* feat);
 hipError_t launch_mlp_infer(hipStream_t st, const float* params, const float* feat, uint32_t n, float* out);
 hipError_t launch_mlp_train(hipStream_t st, const float* params, const float* feat, const float* target, const float* contact,

@@ -252,8 +252,99 @@ __global__ void __launch_bounds__(256) k_gather_states(const uint32_t* __restric
   if (a < n) dst[aosoa_index(i, w)] = src[aosoa_index(a, w)];
 }
 
+// ---- the order by predicted cost ---------------------------------------------------------------------------------------------
+// A stable counting sort on the device: each arena's predicted normalised log cost falls into one of ORDER_BUCKETS
+// buckets (bucket 0 the most expensive); ranks are stable within a bucket (slot order), so the order is deterministic.
+// k_order_count: per-block bucket histograms, stored bucket-major (counts[b * blocks + block]) so that one exclusive scan
+// of the whole array (k_order_scan) gives every (bucket, block) pair its first rank. k_order_place: each arena's rank =
+// that base + its rank among the block's earlier arenas of the same bucket (wave ballots, then the earlier waves' counts),
+// mapped to a slot by the mode.
+__device__ inline uint32_t order_bucket(float p) {
+  float q = (ORDER_HI - p) * ORDER_SCALE;
+  return q != q ? ORDER_BUCKETS - 1u : q <= 0.0f ? 0u : q >= (float)(ORDER_BUCKETS - 1) ? ORDER_BUCKETS - 1u : (uint32_t)q;
+}
+
+__device__ inline uint32_t order_slot(uint32_t s, uint32_t n, int mode) {
+  if (mode != ORDER_BALANCED) return s;
+  // rank s is dealt round-robin over the W waves; the last wave holds r lanes, so after r rounds it is skipped
+  const uint32_t W = (n + AOSOA_LANES - 1) / AOSOA_LANES, r = n - AOSOA_LANES * (W - 1);
+  if (s < r * W) return (s % W) * AOSOA_LANES + s / W;
+  const uint32_t t = s - r * W;  // W > 1 here: with one wave, r * W = n > s
+  return (t % (W - 1)) * AOSOA_LANES + r + t / (W - 1);
+}
+
+__global__ void __launch_bounds__(ORDER_BLOCK) k_order_count(const float* __restrict__ pred, uint32_t n, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[ORDER_BUCKETS];
+  const uint32_t t = threadIdx.x, blocks = gridDim.x;
+  for (uint32_t b = t; b < ORDER_BUCKETS; b += ORDER_BLOCK) h[b] = 0;
+  __syncthreads();
+  const uint32_t a = blockIdx.x * ORDER_BLOCK + t;
+  if (a < n) atomicAdd(&h[order_bucket(pred[2 * (size_t)a])], 1u);
+  __syncthreads();
+  for (uint32_t b = t; b < ORDER_BUCKETS; b += ORDER_BLOCK) counts[(size_t)b * blocks + blockIdx.x] = h[b];
+}
+
+// Exclusive scan of m words in place, one work-group of ORDER_SCAN threads: each thread sums a contiguous chunk, the
+// chunk sums are scanned in LDS, then each thread rewrites its chunk.
+__global__ void __launch_bounds__(ORDER_SCAN) k_order_scan(uint32_t* __restrict__ v, uint32_t m) {
+  __shared__ uint32_t s[ORDER_SCAN];
+  const uint32_t t = threadIdx.x, chunk = (m + ORDER_SCAN - 1) / ORDER_SCAN;
+  const uint32_t lo = t * chunk < m ? t * chunk : m, hi = lo + chunk < m ? lo + chunk : m;
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += v[i];
+  s[t] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < ORDER_SCAN; d <<= 1) {  // Hillis-Steele inclusive scan
+    uint32_t x = t >= d ? s[t - d] : 0u;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  uint32_t run = s[t] - sum;  // exclusive prefix of this chunk
+  for (uint32_t i = lo; i < hi; ++i) {
+    uint32_t c = v[i];
+    v[i] = run;
+    run += c;
+  }
+}
+
+__global__ void __launch_bounds__(ORDER_BLOCK) k_order_place(const float* __restrict__ pred, uint32_t n, const uint32_t* __restrict__ base,
+                                                             int mode, uint32_t* __restrict__ perm) {
+  constexpr uint32_t WAVES = ORDER_BLOCK / AOSOA_LANES;
+  __shared__ uint32_t wc[WAVES][ORDER_BUCKETS];
+  const uint32_t t = threadIdx.x, lane = t % AOSOA_LANES, wave = t / AOSOA_LANES;
+  for (uint32_t e = t; e < WAVES * ORDER_BUCKETS; e += ORDER_BLOCK) wc[e / ORDER_BUCKETS][e % ORDER_BUCKETS] = 0;
+  __syncthreads();
+  const uint32_t a = blockIdx.x * ORDER_BLOCK + t;
+  const bool valid = a < n;
+  const uint32_t b = valid ? order_bucket(pred[2 * (size_t)a]) : 0u;
+  uint64_t peers = __ballot(valid);  // every lane of the wave takes part in each ballot
+  for (uint32_t bit = 0; (1u << bit) < ORDER_BUCKETS; ++bit) {
+    const bool on = (b >> bit) & 1u;
+    const uint64_t m = __ballot(on);
+    peers &= on ? m : ~m;
+  }
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t rank_in_wave = (uint32_t)__popcll(peers & below);
+  if (valid && rank_in_wave == 0) wc[wave][b] = (uint32_t)__popcll(peers);  // the bucket's first lane records its count
+  __syncthreads();
+  if (!valid) return;
+  uint32_t rank = rank_in_wave;
+  for (uint32_t w = 0; w < wave; ++w) rank += wc[w][b];
+  const uint32_t s = base[(size_t)b * gridDim.x + blockIdx.x] + rank;
+  if (s < n) perm[order_slot(s, n, mode)] = a;  // s < n always (the counts sum to n); the check keeps a stray rank out
+}
+
 // ---- launchers -----------------------------------------------------------------------------------------------------------------
 static unsigned blocks_of(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
+uint64_t order_work_words(uint32_t n) { return (uint64_t)ORDER_BUCKETS * blocks_of(n, ORDER_BLOCK); }
+hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int mode, uint32_t* work, uint32_t* perm) {
+  const unsigned blocks = blocks_of(n, ORDER_BLOCK);
+  k_order_count<<<blocks, ORDER_BLOCK, 0, st>>>(pred, n, work);
+  k_order_scan<<<1, ORDER_SCAN, 0, st>>>(work, (uint32_t)order_work_words(n));
+  k_order_place<<<blocks, ORDER_BLOCK, 0, st>>>(pred, n, work, mode, perm);
+  return hipGetLastError();
+}
 hipError_t launch_features(hipStream_t st, const uint32_t* states, uint32_t n, float* feat) {
   k_features<<<blocks_of(n, 256), 256, 0, st>>>(states, n, feat);
   return hipGetLastError();

@@ -41,6 +41,11 @@ struct rsim_words {
   uint32_t* d;
   uint64_t n;
 };
+struct rsim_graph {
+  rsim_device* dev;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
 
 namespace {
 
@@ -492,6 +497,19 @@ int rsim_gather_states(rsim_device* dev, const rsim_states* src, const rsim_word
   RSIM_HIP(rs::dev::launch_gather_states(dev->stream, src->d, perm->d, dst->d, src->n), "gather states");
   return RSIM_OK;
 }
+uint64_t rsim_cost_order_work_words(uint32_t n) { return rs::dev::order_work_words(n); }
+int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* work, rsim_words* perm) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!pred || !work || !perm) return fail(RSIM_ERR_INVALID, "rsim_cost_order: null argument");
+  if (mode != RSIM_ORDER_SORTED && mode != RSIM_ORDER_BALANCED) return fail(RSIM_ERR_INVALID, "rsim_cost_order: unknown mode");
+  if (pred->n < 2ull * n || perm->n < n || work->n < rs::dev::order_work_words(n))
+    return fail(RSIM_ERR_SHORT, "rsim_cost_order: buffers too short");
+  if (!n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_cost_order(dev->stream, reinterpret_cast<const float*>(pred->d), n, mode, work->d, perm->d),
+           "cost order");
+  return RSIM_OK;
+}
 int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words* perm, uint32_t n, uint32_t words_per_item,
                        int scatter, rsim_words* dst) {
   int rc = use(dev);
@@ -501,6 +519,45 @@ int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words
     return fail(RSIM_ERR_SHORT, "rsim_permute_words: buffers too short");
   if (!n || !words_per_item) return RSIM_OK;
   RSIM_HIP(rs::dev::launch_gather_words(dev->stream, src->d, perm->d, dst->d, n, words_per_item, scatter != 0), "permute words");
+  return RSIM_OK;
+}
+// ---- HIP graphs ------------------------------------------------------------------------------------------------------------
+int rsim_capture_begin(rsim_device* dev) {
+  int rc = use(dev);
+  if (rc) return rc;
+  RSIM_HIP(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return RSIM_OK;
+}
+int rsim_capture_end(rsim_device* dev, rsim_graph** out) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!out) return fail(RSIM_ERR_INVALID, "rsim_capture_end: null argument");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  RSIM_HIP(hipStreamEndCapture(dev->stream, &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *out = new rsim_graph{dev, g, x};
+  return RSIM_OK;
+}
+int rsim_graph_launch(rsim_device* dev, const rsim_graph* graph) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!graph || graph->dev != dev) return fail(RSIM_ERR_INVALID, "rsim_graph_launch: graph of another device");
+  RSIM_HIP(hipGraphLaunch(graph->exec, dev->stream), "hipGraphLaunch");
+  return RSIM_OK;
+}
+int rsim_graph_free(rsim_graph* graph) {
+  if (!graph) return RSIM_OK;
+  (void)hipSetDevice(graph->dev->ordinal);
+  (void)hipStreamSynchronize(graph->dev->stream);
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
   return RSIM_OK;
 }
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {

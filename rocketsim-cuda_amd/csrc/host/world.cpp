@@ -887,6 +887,8 @@ int scenario_of(int scenario, uint64_t pick) {
       uint32_t p = (uint32_t)(pick % 3);
       return p == 0 ? rs::world::SCEN_KICKOFF : p == 1 ? rs::world::SCEN_OPEN : scenario_of(RSIM_SCEN_CONTACT, pick / 3);
     }
+    case RSIM_SCEN_OPEN_CONTACT:  // 1:1 open play, contact-heavy
+      return (pick & 1) == 0 ? rs::world::SCEN_OPEN : scenario_of(RSIM_SCEN_CONTACT, pick >> 1);
     default:
       return scenario;
   }
@@ -908,7 +910,7 @@ int rsim_build_world(int preset0, int preset1, float gravity_z, float boost_acce
 }
 
 int rsim_sample_arenas(const void* world, int scenario, uint64_t seed, uint32_t first, uint32_t n, void* states_out) {
-  if (!world || !states_out || scenario < 0 || scenario > RSIM_SCEN_MIXED) return RSIM_ERR_INVALID;
+  if (!world || !states_out || scenario < 0 || scenario > RSIM_SCEN_OPEN_CONTACT) return RSIM_ERR_INVALID;
   const rs::StaticData& w = *(const rs::StaticData*)world;
   rs::ArenaState* out = (rs::ArenaState*)states_out;
   for (uint32_t i = 0; i < n; ++i) {

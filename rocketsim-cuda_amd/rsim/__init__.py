@@ -35,7 +35,7 @@ PHASE_OK, PHASE_UNIMPLEMENTED, PHASE_BAD_INDEX = 0, -1, -2
 
 NUM_PRESETS = 7
 PRESETS = ("octane", "dominus", "plank", "breakout", "hybrid", "merc", "psyclops")
-SCEN_KICKOFF, SCEN_OPEN, SCEN_PINCH, SCEN_HEADON, SCEN_GOALFRAME, SCEN_CONTACT, SCEN_MIXED = range(7)
+SCEN_KICKOFF, SCEN_OPEN, SCEN_PINCH, SCEN_HEADON, SCEN_GOALFRAME, SCEN_CONTACT, SCEN_MIXED, SCEN_OPEN_CONTACT = range(8)
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -69,6 +69,10 @@ def lib():
             "rsim_cost_gradients": ([_vp, _vp, _vp, _vp, _vp, u32, _vp, _vp, _vp], ctypes.c_int),
             "rsim_cost_adam": ([_vp, _vp, _vp, _vp, _vp, f32, u32], ctypes.c_int),
             "rsim_set_permutation": ([_vp, _vp, u32, _vp], ctypes.c_int),
+            "rsim_cost_order_work_words": ([u32], u64),
+            "rsim_cost_order": ([_vp, _vp, u32, ctypes.c_int, _vp, _vp], ctypes.c_int),
+            "rsim_capture_begin": ([_vp], ctypes.c_int), "rsim_capture_end": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
+            "rsim_graph_launch": ([_vp, _vp], ctypes.c_int), "rsim_graph_free": ([_vp], ctypes.c_int),
             "rsim_gather_states": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "rsim_permute_words": ([_vp, _vp, _vp, u32, u32, ctypes.c_int, _vp], ctypes.c_int),
             "rsim_upload_states": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -173,7 +177,7 @@ class DeviceStatic:
         self.dev, self.handle = dev, handle
 
     def free(self):
-        if self.handle:
+        if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
             lib().rsim_world_free(self.handle)
             self.handle = None
 
@@ -195,8 +199,29 @@ class DeviceStates:
         return lib().rsim_states_device_ptr(self.handle)
 
     def free(self):
-        if self.handle:
+        if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
             lib().rsim_states_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A recorded sequence of asynchronous calls (a HIP graph), replayed with `launch`."""
+
+    def __init__(self, dev, handle):
+        self.dev, self.handle = dev, handle
+
+    def launch(self):
+        _check(lib().rsim_graph_launch(self.dev.handle, self.handle), "rsim_graph_launch")
+
+    def free(self):
+        if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
+            lib().rsim_graph_free(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -217,7 +242,7 @@ class DeviceWords:
         return lib().rsim_words_device_ptr(self.handle)
 
     def free(self):
-        if self.handle:
+        if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
             lib().rsim_words_free(self.handle)
             self.handle = None
 
@@ -354,6 +379,16 @@ class SimDevice:
 
     def sync(self):
         _check(lib().rsim_sync(self.handle), "rsim_sync")
+
+    def capture_begin(self):
+        """Start recording the asynchronous calls on this device's stream into a HIP graph."""
+        _check(lib().rsim_capture_begin(self.handle), "rsim_capture_begin")
+
+    def capture_end(self):
+        """Stop recording; returns the instantiated Graph."""
+        h = _vp()
+        _check(lib().rsim_capture_end(self.handle, ctypes.byref(h)), "rsim_capture_end")
+        return Graph(self, h)
 
     def timer_begin(self):
         _check(lib().rsim_timer_begin(self.handle), "rsim_timer_begin")

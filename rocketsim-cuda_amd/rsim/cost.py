@@ -123,37 +123,66 @@ def label_costs(dev, world, states, controls, decisions=1):
     return np.concatenate(feats), np.concatenate(targets), np.concatenate(contacts)
 
 
+ORDER_BUCKETS, ORDER_HI, ORDER_SCALE = 256, np.float32(3.0), np.float32(64.0)
+ORDER_MODES = {"sorted": 0, "balanced": 1}
+
+
+def cost_buckets(pred_cost):
+    """The device's bucket of each predicted normalised log cost (0 = most expensive; NaN in the last)."""
+    p = np.asarray(pred_cost, np.float32)
+    q = (ORDER_HI - p) * ORDER_SCALE  # float32 arithmetic, as on the device
+    b = np.where(np.isnan(q), ORDER_BUCKETS - 1, 0).astype(np.int64)
+    ok = ~np.isnan(q)
+    b[ok] = np.where(q[ok] <= 0, 0, np.where(q[ok] >= ORDER_BUCKETS - 1, ORDER_BUCKETS - 1, np.floor(np.maximum(q[ok], 0))))
+    return b.astype(np.uint32)
+
+
+def deal_slots(n, wave=64):
+    """Slot of each rank in 'balanced' order: ranks dealt round-robin over the ceil(n / wave) waves; the last (partial)
+    wave of r lanes takes part in the first r rounds only."""
+    W = (n + wave - 1) // wave
+    r = n - wave * (W - 1)
+    s = np.arange(n, dtype=np.int64)
+    slot = (s % W) * wave + s // W
+    if W > 1:
+        late = s >= r * W
+        t = s[late] - r * W
+        slot[late] = (t % (W - 1)) * wave + r + t // (W - 1)
+    return slot
+
+
 def order_by_cost(pred_cost, mode, wave=64):
-    """Arena order from predicted costs: 'sorted' (cost-homogeneous waves, most expensive first) or 'balanced' (the sorted
-    arenas dealt round-robin over the waves, so every wave gets one of each cost rank). Returns perm: slot i <- arena perm[i]."""
-    n = pred_cost.shape[0]
-    srt = np.argsort(-pred_cost, kind="stable").astype(np.uint32)
+    """The host statement of rsim_cost_order: perm[i] = the current slot that moves to slot i, by bucketed predicted cost
+    (most expensive first, stable) in 'sorted' mode, or those ranks dealt over the waves in 'balanced' mode."""
+    if mode not in ORDER_MODES:
+        raise ValueError(mode)
+    srt = np.argsort(cost_buckets(pred_cost), kind="stable").astype(np.uint32)
     if mode == "sorted":
         return srt
-    if mode == "balanced":
-        waves = (n + wave - 1) // wave
-        slots = np.arange(n)
-        # rank r goes to wave r % waves, lane r // waves
-        dest = (slots % waves) * wave + slots // waves
-        perm = np.empty(n, np.uint32)
-        order = np.argsort(dest, kind="stable")
-        perm[:] = srt[order]
-        return perm
-    raise ValueError(mode)
+    perm = np.empty_like(srt)
+    perm[deal_slots(srt.shape[0], wave)] = srt
+    return perm
 
 
 class Regrouper:
-    """Steps decisions of n arenas in a cost-regrouped order: the arenas live in a work buffer in the current order; each
-    decision re-orders them by the model's prediction (gather), gathers the controls, steps, and scatters the result words
-    back to arena order. `states_in_arena_order` un-permutes."""
+    """Steps decisions of n arenas in a cost-regrouped order, entirely on the device (no host synchronisation, so a
+    decision can be captured in a HIP graph). The arenas live in a work buffer in the current order, slot i holding arena
+    where[i]. Each regroup: features -> MLP -> rsim_cost_order -> gather the states into the new order and compose
+    `where`. Each step gathers the controls by `where`, runs sim8 and scatters the result words back to arena order."""
 
     def __init__(self, dev, model, world, states, mode="sorted", block=SIM8_BLOCK):
+        if mode not in ORDER_MODES:
+            raise ValueError(mode)
         self.dev, self.model, self.world, self.mode, self.block = dev, model, world, mode, block
         self.n = n = states.n
-        self.cur = states  # work buffer, slot i holds arena self.where[i]
+        self.cur = states  # the work buffer (the caller's states buffer is used and reordered)
         self.spare = dev.alloc_states(n)
-        self.where = np.arange(n, dtype=np.uint32)
-        self.perm = dev.alloc_words(n)
+        iota = np.arange(n, dtype=np.uint32)
+        self.iota = dev.upload_words(iota)
+        self.where = dev.upload_words(iota)
+        self.where2 = dev.alloc_words(n)
+        self.rel = dev.alloc_words(n)
+        self.work = dev.alloc_words(int(lib().rsim_cost_order_work_words(n)))
         self.feat = dev.alloc_words(n * FEATURES)
         self.pred = dev.alloc_words(2 * n)
         self.ctl = dev.alloc_words(n * CONTROL_WORDS)
@@ -163,28 +192,29 @@ class Regrouper:
         n = self.n
         self.model.features(self.cur, self.feat)
         self.model.predict(self.feat, n, self.pred)
-        cost = self.dev.read_words(self.pred).view(np.float32).reshape(n, 2)[:, 0]
-        rel = order_by_cost(cost, self.mode)  # new slot i <- current slot rel[i]
-        _check(lib().rsim_set_permutation(self.dev.handle, _ptr(np.ascontiguousarray(rel)), n, self.perm.handle),
-               "rsim_set_permutation")
-        _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, self.perm.handle, self.spare.handle), "rsim_gather_states")
+        _check(lib().rsim_cost_order(self.dev.handle, self.pred.handle, n, ORDER_MODES[self.mode], self.work.handle,
+                                     self.rel.handle), "rsim_cost_order")
+        _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, self.rel.handle, self.spare.handle), "rsim_gather_states")
         self.cur, self.spare = self.spare, self.cur
-        self.where = self.where[rel]
-        _check(lib().rsim_set_permutation(self.dev.handle, _ptr(self.where), n, self.perm.handle), "rsim_set_permutation")
+        _check(lib().rsim_permute_words(self.dev.handle, self.where.handle, self.rel.handle, n, 1, 0, self.where2.handle),
+               "rsim_permute_words")
+        self.where, self.where2 = self.where2, self.where
 
     def step(self, controls, result):
         """One decision: controls (DeviceWords, arena order) and result (DeviceWords, arena order)."""
         n = self.n
-        _check(lib().rsim_permute_words(self.dev.handle, controls.handle, self.perm.handle, n, CONTROL_WORDS, 0, self.ctl.handle),
+        _check(lib().rsim_permute_words(self.dev.handle, controls.handle, self.where.handle, n, CONTROL_WORDS, 0, self.ctl.handle),
                "rsim_permute_words")
         self.dev.sim8(self.cur, self.world, self.ctl, self.res, self.block)
-        _check(lib().rsim_permute_words(self.dev.handle, self.res.handle, self.perm.handle, n, 1, 1, result.handle),
+        _check(lib().rsim_permute_words(self.dev.handle, self.res.handle, self.where.handle, n, 1, 1, result.handle),
                "rsim_permute_words")
 
-    def states_in_arena_order(self):
-        inv = np.empty(self.n, np.uint32)
-        inv[self.where] = np.arange(self.n, dtype=np.uint32)
-        _check(lib().rsim_set_permutation(self.dev.handle, _ptr(inv), self.n, self.perm.handle), "rsim_set_permutation")
-        out = self.dev.alloc_states(self.n)
-        _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, self.perm.handle, out.handle), "rsim_gather_states")
+    def states_in_arena_order(self, out=None):
+        """A states buffer with arena a at position a (asynchronous)."""
+        inv = self.dev.alloc_words(self.n)
+        _check(lib().rsim_permute_words(self.dev.handle, self.iota.handle, self.where.handle, self.n, 1, 1, inv.handle),
+               "rsim_permute_words")
+        out = out if out is not None else self.dev.alloc_states(self.n)
+        _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, inv.handle, out.handle), "rsim_gather_states")
+        self.dev.sync()
         return out

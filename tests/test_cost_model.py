@@ -68,14 +68,27 @@ def test_reference_gradient_matches_finite_differences():
 
 
 def test_order_by_cost():
+    """The host statement of the device order, against brute force: a stable sort by (bucket, slot), and the balanced deal
+    puts one arena of every cost rank in each wave."""
     rng = np.random.default_rng(2)
-    cost = rng.standard_normal(1000).astype(np.float32)
-    s = cm.order_by_cost(cost, "sorted")
-    assert sorted(s.tolist()) == list(range(1000)) and (np.diff(cost[s]) <= 0).all()
-    b = cm.order_by_cost(cost, "balanced")
-    assert sorted(b.tolist()) == list(range(1000))
-    waves = [cost[b[i:i + 64]] for i in range(0, 1000, 64)]
-    assert max(w.max() for w in waves) - min(w.max() for w in waves[:-1]) < max(cost) - np.median(cost)  # spread out
+    for n in (1, 5, 63, 64, 65, 1000, 4099):
+        cost = (rng.integers(-40, 240, n) / 64.0).astype(np.float32)  # many ties, values past both ends of the range
+        cost[rng.random(n) < 0.02] = np.nan
+        b = cm.cost_buckets(cost)
+        for i in rng.choice(n, min(n, 50), replace=False):  # the bucket from its definition
+            c = float(cost[i])
+            want = 255 if np.isnan(c) else int(min(255, max(0, np.floor((3.0 - c) * 64.0))))
+            assert b[i] == want, (c, b[i], want)
+        s = cm.order_by_cost(cost, "sorted")
+        assert s.tolist() == sorted(range(n), key=lambda i: (int(b[i]), i))
+        bal = cm.order_by_cost(cost, "balanced")
+        assert sorted(bal.tolist()) == list(range(n))
+        rank = np.empty(n, np.int64)
+        rank[s] = np.arange(n)
+        W = (n + 63) // 64
+        for w in range(W):  # wave w holds ranks w, w + W, w + 2W, ... (the partial last wave: its first rounds only)
+            r = np.sort(rank[bal[w * 64:(w + 1) * 64]])
+            assert r[0] == w and (n <= 64 or (np.diff(r) <= W).all()), (n, w, r[:4])
 
 
 # ---- the kernels --------------------------------------------------------------------------------------------------------
@@ -176,3 +189,118 @@ def test_features_and_labels(rsim, world, device):
     assert np.allclose(feat[:, 0], np.clip(bp[:, 0] / 4096.0, -4, 4), rtol=1e-6)
     assert np.isfinite(target).all() and target.std() > 0  # arenas differ in cost
     assert set(np.unique(contact)) <= {0.0, 1.0}
+
+
+@pytest.mark.gpu
+def test_device_order_matches_host(rsim, device):
+    note("rsim_cost_order vs its host statement, sizes 1..70001, both modes")
+    rng = np.random.default_rng(11)
+    for n in (1, 63, 65, 1000, 4099, 70001):
+        pred = np.zeros((n, 2), np.float32)
+        pred[:, 0] = (rng.integers(-40, 240, n) / 64.0).astype(np.float32) + rng.random(n).astype(np.float32) * 0.01
+        pred[rng.random(n) < 0.01, 0] = np.nan
+        d_pred = device.upload_words(pred.view(np.uint32).ravel())
+        work = device.alloc_words(int(rsim.lib().rsim_cost_order_work_words(n)))
+        perm = device.alloc_words(n)
+        for mode, code in cm.ORDER_MODES.items():
+            rsim._check(rsim.lib().rsim_cost_order(device.handle, d_pred.handle, n, code, work.handle, perm.handle), "order")
+            device.sync()
+            got = device.read_words(perm)
+            want = cm.order_by_cost(pred[:, 0], mode)
+            assert np.array_equal(got, want), (n, mode, np.flatnonzero(got != want)[:5])
+
+
+@pytest.mark.gpu
+def test_graph_replay_matches_eager(rsim, world, device, dev_world):
+    note("3 sim8 decisions captured in a HIP graph, replayed 3 times vs eager, 2048 arenas")
+    n = 2048
+    states = rsim.sample_arenas(world, rsim.SCEN_CONTACT, 151, n)
+    ctl = [device.upload_controls(rsim.sample_controls(152, k, n)) for k in range(3)]
+    init = device.upload_states(states)
+    work = device.alloc_states(n)
+    res = [device.alloc_words(n) for _ in range(3)]
+    device.copy_states(work, init)
+    for k in range(3):
+        device.sim8(work, dev_world, ctl[k], res[k])
+    device.sync()
+    want_res = [device.read_words(r) for r in res]
+    want = device.download_states(work)
+    device.capture_begin()
+    for k in range(3):
+        device.sim8(work, dev_world, ctl[k], res[k])
+    graph = device.capture_end()
+    for rep in range(3):
+        for r in res:
+            device.write_words(r, np.zeros(n, np.uint32))
+        device.copy_states(work, init)  # eager work between replays
+        device.sync()
+        graph.launch()
+        device.sync()
+        for k in range(3):
+            assert np.array_equal(device.read_words(res[k]), want_res[k]), (rep, k)
+        assert np.array_equal(device.download_states(work)[:, :2567], want[:, :2567]), rep
+    graph.free()
+
+
+@pytest.mark.gpu
+def test_regrouper_in_a_graph(rsim, world, device, dev_world):
+    note("two regrouped decisions captured in a graph, replayed twice (4 decisions) vs plain, 4096 arenas")
+    n = 4096
+    states = rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 161, n)
+    ctl = [rsim.sample_controls(162, k, n) for k in range(4)]
+    plain = device.upload_states(states)
+    r = device.alloc_words(n)
+    want_res = []
+    for c in ctl:
+        device.sim8(plain, dev_world, device.upload_controls(c), r)
+        device.sync()
+        want_res.append(device.read_words(r))
+    want = device.download_states(plain)
+    model = cm.CostModel(device, seed=12)
+    for mode in ("sorted", "balanced"):
+        g = cm.Regrouper(device, model, dev_world, device.upload_states(states), mode)
+        stage = [device.alloc_words(n * rsim.CONTROL_WORDS) for _ in range(2)]
+        out = [device.alloc_words(n) for _ in range(2)]
+        device.sync()
+        device.capture_begin()
+        for k in range(2):  # an even number of decisions: the double buffers are back in place after each replay
+            g.regroup()
+            g.step(stage[k], out[k])
+        graph = g.dev.capture_end()
+        for rep in range(2):
+            for k in range(2):
+                device.write_words(stage[k], ctl[2 * rep + k])
+            graph.launch()
+            device.sync()
+            for k in range(2):
+                assert np.array_equal(device.read_words(out[k]), want_res[2 * rep + k]), (mode, rep, k)
+        got = device.download_states(g.states_in_arena_order())
+        assert np.array_equal(got[:, :2567], want[:, :2567]), mode
+        graph.free()
+
+
+@pytest.mark.gpu
+def test_trained_model_predicts_cost_better_than_untrained(rsim, world, device, dev_world):
+    """What training gains: on held-out arenas, the correlation of predicted and measured log cost (lane clocks of the
+    simulator itself, one arena per wave) rises well above the untrained network's."""
+    note("cost model trained on simulator labels vs untrained, held-out correlation")
+    n = 4096
+    f, t, c = cm.label_costs(device, dev_world, rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 171, n),
+                             [rsim.sample_controls(172, d, n) for d in range(4)], decisions=4)
+    hf, ht, _ = cm.label_costs(device, dev_world, rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 173, 2048),
+                               [rsim.sample_controls(174, d, 2048) for d in range(2)], decisions=2)
+    d_hf = device.upload_words(hf.view(np.uint32).ravel())
+
+    def corr(model):
+        out = model.predict(d_hf, hf.shape[0])
+        device.sync()
+        p = device.read_words(out, 2 * hf.shape[0]).view(np.float32).reshape(-1, 2)[:, 0]
+        return float(np.corrcoef(p, ht)[0, 1])
+
+    untrained = cm.CostModel(device, seed=13)
+    before = corr(untrained)
+    model = cm.CostModel(device, seed=13)
+    curve = model.train(f, t, c, epochs=15, batch=2048, lr=3e-3)
+    after = corr(model)
+    print("[case] held-out pearson: untrained %.3f trained %.3f, loss %.3f -> %.3f" % (before, after, curve[0], curve[-1]))
+    assert after > 0.35 and after > before + 0.2, (before, after)
