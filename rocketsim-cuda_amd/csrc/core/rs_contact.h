@@ -6,11 +6,14 @@
 
 namespace rs {
 
-// The narrowphase context: the arena, its world, and a count of pair algorithms the tick reached that are not implemented.
+struct SatBatch;
+// The narrowphase context: the arena, its world, a count of pair algorithms the tick reached that are not implemented, and
+// the tick's precomputed separating-axis tests of the car-mesh jobs (null: every job tests its triangles in place).
 struct Np {
   ArenaState& s;
   const StaticData& w;
   uint32_t unimplemented;
+  SatBatch* sb;
 };
 RS_FN Aff body_trans(const Np& c, uint32_t b) {
   int d = dyn_slot(c.w, b);

@@ -24,6 +24,21 @@
 #define RS_TABLE static const
 #endif
 
+// Address-space qualifiers for the gfx950 compiler. A non-inlined function that takes its lane-private state and its global
+// StaticData through pointers qualified with these compiles their accesses to scratch_* / global_* instructions; through
+// generic pointers they would be flat instructions, which wait on both the vector-memory and the LDS/scalar counters and
+// need 64-bit VGPR addresses. Empty on the host.
+// RS_PRIVATE(p) states the same for a pointer parameter that always points into the lane's private memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RS_PRIVATE_AS __attribute__((address_space(5)))
+#define RS_GLOBAL_AS __attribute__((address_space(1)))
+#define RS_PRIVATE(p) __builtin_assume(__builtin_amdgcn_is_private((const void*)(p)))
+#else
+#define RS_PRIVATE_AS
+#define RS_GLOBAL_AS
+#define RS_PRIVATE(p) ((void)0)
+#endif
+
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #define RS_UNROLL _Pragma("unroll")

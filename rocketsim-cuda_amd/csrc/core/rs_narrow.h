@@ -486,6 +486,175 @@ RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
 }
 
 // ---- a car against an arena mesh (compound_collision_alg.cuh:63-160) --------------------------------------------------------------
+// The job's frames: the car box in world space, the mesh body's transform, the box's AABB in mesh space, the mesh-to-box
+// transform the triangles are tested in, and the contact distance limit.
+struct CarMeshFrame {
+  Aff box_t, mt, mesh_to_box;
+  Aabb local_box;
+  float maxd;
+};
+RS_FN CarMeshFrame car_mesh_frame(const Np& c, uint32_t car, uint32_t mesh_body) {
+  const BodyConst& bc = c.w.bodies[car];
+  CarMeshFrame f;
+  f.box_t = child_world(c, car);
+  f.mt = body_trans(c, mesh_body);
+  Aff x1 = aff_transpose(f.mt);
+  Aff in_tri = Aff{x1.m * f.box_t.m, xform_point(x1, f.box_t.t)};
+  f.local_box = box_from_half_extents(bc.box_half_extents, bc.box_margin, in_tri);
+  f.mesh_to_box = aff_transpose(f.box_t) * f.mt;
+  float breaking = fmin_rs(bc.contact_breaking_threshold, c.w.bodies[mesh_body].contact_breaking_threshold);
+  f.maxd = bc.box_margin + breaking;
+  return f;
+}
+RS_FN bool tri_in_box(const Tri& t, const Aabb& b) {
+  V3 tmin = vmin(vmin(t.p0, t.p1), t.p2), tmax = vmax(vmax(t.p0, t.p1), t.p2);
+  return all3(cmple(tmin, b.max)) && all3(cmpge(tmax, b.min));
+}
+
+// The tick's car-mesh separating-axis tests, run ahead of the pair loop. P3b first walks the pairs that will reach
+// car_mesh (the same gate as near_callback) and collects each job's candidate triangles in BVH order; the tests then run
+// over all collected triangles at once (on the GPU spread over every active lane of the wave, so a lane with 30
+// triangles no longer holds the other 63 for 30 tests); car_mesh then adds the contacts in the original order. Results
+// are the same: a test depends only on its box, triangle and limits, and nothing in P3b changes those. A job that does
+// not fit (more than SB_JOBS jobs or SB_TASKS triangles in the tick) is left to car_mesh's own loop.
+constexpr int SB_JOBS = 4, SB_TASKS = 32;
+struct SatJob {
+  uint32_t car, mesh_body;
+  uint32_t start, count;
+  bool in_place;  // the triangles did not fit: car_mesh walks and tests them itself
+  Aff box_t, mesh_to_box;
+  V3 half;
+  float margin, maxd;
+  uint32_t tri_first;
+};
+struct SatBatch {
+  uint32_t n_jobs, n_tasks, next_job, ok;  // ok: bit t set when task t's test found a contact
+  SatJob jobs[SB_JOBS];
+  uint32_t tri[SB_TASKS];
+  uint8_t job_of[SB_TASKS];
+  SatContact res[SB_TASKS];
+};
+constexpr int SB_RECORD = 26;  // a staged test: box transform (12), half extents (3), margin, limit, triangle (9)
+
+RS_FN void sat_collect(const Np& c, uint32_t car, uint32_t mesh_body, SatBatch& sb) {
+  if (sb.n_jobs >= (uint32_t)SB_JOBS) return;
+  const MeshInfo* mesh = mesh_of(c.w, mesh_body);
+  if (mesh == nullptr) return;
+  const BodyConst& bc = c.w.bodies[car];
+  CarMeshFrame f = car_mesh_frame(c, car, mesh_body);
+  SatJob& j = sb.jobs[sb.n_jobs];
+  j.car = car;
+  j.mesh_body = mesh_body;
+  j.start = sb.n_tasks;
+  j.in_place = false;
+  j.box_t = f.box_t;
+  j.mesh_to_box = f.mesh_to_box;
+  j.half = bc.box_half_extents;
+  j.margin = bc.box_margin;
+  j.maxd = f.maxd;
+  j.tri_first = mesh->tri_first;
+  const Tri* tris = c.w.tris + mesh->tri_first;
+  BvhCursor cur;
+  cursor_init(cur, *mesh, f.local_box);
+  uint32_t tri;
+  while (cursor_next(cur, c.w, *mesh, f.local_box, &tri)) {
+    if (!tri_in_box(tris[tri], f.local_box)) continue;
+    if (sb.n_tasks >= (uint32_t)SB_TASKS) {
+      j.in_place = true;
+      sb.n_tasks = j.start;
+      break;
+    }
+    sb.tri[sb.n_tasks] = tri;
+    sb.job_of[sb.n_tasks] = (uint8_t)sb.n_jobs;
+    ++sb.n_tasks;
+  }
+  j.count = sb.n_tasks - j.start;
+  ++sb.n_jobs;
+}
+// The staged record of task t (its triangle transformed into the box frame, as car_mesh does).
+RS_FN void sat_record(const Np& c, const SatBatch& sb, uint32_t t, float* r) {
+  const SatJob& j = sb.jobs[sb.job_of[t]];
+  const Tri& tr = c.w.tris[j.tri_first + sb.tri[t]];
+  V3 q[3] = {xform_point(j.mesh_to_box, tr.p0), xform_point(j.mesh_to_box, tr.p1), xform_point(j.mesh_to_box, tr.p2)};
+  const V3 v[8] = {j.box_t.m.c0, j.box_t.m.c1, j.box_t.m.c2, j.box_t.t, j.half, q[0], q[1], q[2]};
+  for (int i = 0; i < 5; ++i) {
+    r[3 * i] = v[i].x;
+    r[3 * i + 1] = v[i].y;
+    r[3 * i + 2] = v[i].z;
+  }
+  r[15] = j.margin;
+  r[16] = j.maxd;
+  for (int i = 0; i < 3; ++i) {
+    r[17 + 3 * i] = v[5 + i].x;
+    r[18 + 3 * i] = v[5 + i].y;
+    r[19 + 3 * i] = v[5 + i].z;
+  }
+}
+RS_FN bool sat_run_record(const float* r, SatContact* out) {
+  Aff box_t = Aff{M3{v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), v3(r[6], r[7], r[8])}, v3(r[9], r[10], r[11])};
+  V3 half = v3(r[12], r[13], r[14]);
+  V3 q[3] = {v3(r[17], r[18], r[19]), v3(r[20], r[21], r[22]), v3(r[23], r[24], r[25])};
+  return box_triangle_sat(box_t, half, r[15], q, r[16], out);
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+// All active lanes of the wave take the collected tests of all of them in turns of one test each: lane ranks and task
+// offsets from ballots, the records staged in the lanes' LDS slots, the results handed back the same way.
+__device__ inline void sat_batch_run(const Np& c, SatBatch& sb) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+  const uint64_t active = __ballot(1);
+  const uint32_t nact = (uint32_t)__popcll(active), rank = (uint32_t)__popcll(active & below);
+  const uint32_t cnt = sb.n_tasks;
+  uint32_t pre = 0, total = 0;
+  for (uint32_t bit = 0; (1u << bit) <= (uint32_t)SB_TASKS; ++bit) {
+    const uint64_t m = __ballot((cnt >> bit) & 1u);
+    pre += (uint32_t)__popcll(m & below) << bit;
+    total += (uint32_t)__popcll(m) << bit;
+  }
+  sb.ok = 0;
+  for (uint32_t base = 0; base < total; base += nact) {  // total and nact are the same in every lane
+    const uint32_t lo = (pre > base ? pre : base) - pre;
+    const uint32_t end = pre + cnt < base + nact ? pre + cnt : base + nact;
+    const uint32_t hi = end > pre ? end - pre : 0u;
+    float r[SB_RECORD];
+    for (uint32_t t = lo; t < hi; ++t) {  // this lane's tasks in the window, into the slots their order gives
+      sat_record(c, sb, t, r);
+      const uint32_t slot = pre + t - base;
+      for (int k = 0; k < SB_RECORD; ++k) *wave_slot(slot, (uint32_t)k) = r[k];
+    }
+    wave_lds_sync();
+    if (base + rank < total) {
+      for (int k = 0; k < SB_RECORD; ++k) r[k] = *wave_slot(rank, (uint32_t)k);
+      SatContact ct = SatContact{};
+      const bool hit = sat_run_record(r, &ct);
+      *wave_slot(rank, 0) = hit ? 1.0f : 0.0f;
+      *wave_slot(rank, 1) = ct.point_world.x;
+      *wave_slot(rank, 2) = ct.point_world.y;
+      *wave_slot(rank, 3) = ct.point_world.z;
+      *wave_slot(rank, 4) = ct.distance;
+    }
+    wave_lds_sync();
+    for (uint32_t t = lo; t < hi; ++t) {
+      const uint32_t slot = pre + t - base;
+      if (*wave_slot(slot, 0) != 0.0f) sb.ok |= 1u << t;
+      sb.res[t] = SatContact{v3(*wave_slot(slot, 1), *wave_slot(slot, 2), *wave_slot(slot, 3)), *wave_slot(slot, 4)};
+    }
+    wave_lds_sync();
+  }
+}
+#else
+inline void sat_batch_run(const Np& c, SatBatch& sb) {  // one lane: the tests in order
+  sb.ok = 0;
+  float r[SB_RECORD];
+  for (uint32_t t = 0; t < sb.n_tasks; ++t) {
+    sat_record(c, sb, t, r);
+    SatContact ct = SatContact{};
+    if (sat_run_record(r, &ct)) sb.ok |= 1u << t;
+    sb.res[t] = ct;
+  }
+}
+#endif
+
 RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
   const MeshInfo* mesh = mesh_of(c.w, mesh_body);
   if (mesh == nullptr) {
@@ -493,29 +662,38 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
     return false;
   }
   const BodyConst& bc = c.w.bodies[car];
-  Aff box_t = child_world(c, car);
-  Aff mt = body_trans(c, mesh_body);
-  Aff x1 = aff_transpose(mt);
-  Aff in_tri = Aff{x1.m * box_t.m, xform_point(x1, box_t.t)};
-  Aabb local_box = box_from_half_extents(bc.box_half_extents, bc.box_margin, in_tri);
-  Aff mesh_to_box = aff_transpose(box_t) * mt;
-  float breaking = fmin_rs(bc.contact_breaking_threshold, c.w.bodies[mesh_body].contact_breaking_threshold);
-  float maxd = bc.box_margin + breaking;
+  CarMeshFrame f = car_mesh_frame(c, car, mesh_body);
   const Tri* tris = c.w.tris + mesh->tri_first;
   Manifold m = manifold_new(car, mesh_body);
   bool has_m = false;
-  BvhCursor cur;
-  cursor_init(cur, *mesh, local_box);
-  uint32_t tri;
-  while (cursor_next(cur, c.w, *mesh, local_box, &tri)) {
-    const Tri& t = tris[tri];
-    V3 tmin = vmin(vmin(t.p0, t.p1), t.p2), tmax = vmax(vmax(t.p0, t.p1), t.p2);
-    if (!(all3(cmple(tmin, local_box.max)) && all3(cmpge(tmax, local_box.min)))) continue;
-    V3 q[3] = {xform_point(mesh_to_box, t.p0), xform_point(mesh_to_box, t.p1), xform_point(mesh_to_box, t.p2)};
-    SatContact ct;
-    if (!box_triangle_sat(box_t, bc.box_half_extents, bc.box_margin, q, maxd, &ct)) continue;
-    has_m = true;  // the manifold exists from the first contact on
-    add_contact(c, m, car, mesh_body, mt.m * t.normal, ct.point_world, ct.distance, (int32_t)tri);
+  const SatJob* job = nullptr;
+  if (c.sb)
+    for (uint32_t k = c.sb->next_job; k < c.sb->n_jobs; ++k)
+      if (c.sb->jobs[k].car == car && c.sb->jobs[k].mesh_body == mesh_body) {
+        job = &c.sb->jobs[k];
+        c.sb->next_job = k + 1;
+        break;
+      }
+  if (job && !job->in_place) {  // the tests ran ahead: add the contacts in BVH order
+    for (uint32_t t = job->start; t < job->start + job->count; ++t) {
+      if (!((c.sb->ok >> t) & 1u)) continue;
+      const uint32_t tri = c.sb->tri[t];
+      has_m = true;  // the manifold exists from the first contact on
+      add_contact(c, m, car, mesh_body, f.mt.m * tris[tri].normal, c.sb->res[t].point_world, c.sb->res[t].distance, (int32_t)tri);
+    }
+  } else {
+    BvhCursor cur;
+    cursor_init(cur, *mesh, f.local_box);
+    uint32_t tri;
+    while (cursor_next(cur, c.w, *mesh, f.local_box, &tri)) {
+      const Tri& t = tris[tri];
+      if (!tri_in_box(t, f.local_box)) continue;
+      V3 q[3] = {xform_point(f.mesh_to_box, t.p0), xform_point(f.mesh_to_box, t.p1), xform_point(f.mesh_to_box, t.p2)};
+      SatContact ct;
+      if (!box_triangle_sat(f.box_t, bc.box_half_extents, bc.box_margin, q, f.maxd, &ct)) continue;
+      has_m = true;
+      add_contact(c, m, car, mesh_body, f.mt.m * t.normal, ct.point_world, ct.distance, (int32_t)tri);
+    }
   }
   if (!has_m) return false;
   refresh_points(c, m, car, mesh_body);
@@ -613,10 +791,33 @@ RS_FN void near_callback(Np& c, uint32_t a, uint32_t b) {
   else has_fresh = process_inner(c, a, b, &fresh);
   near_tail(c, a, b, cached, has_fresh, fresh);
 }
+// Whether near_callback(a, b) reaches car_mesh, and with which car and mesh: its checks before that call, none of which
+// depends on anything P3b writes (activation, collision flags, transforms, shapes).
+RS_FN bool reaches_car_mesh(const Np& c, uint32_t a, uint32_t b, uint32_t* car, uint32_t* mesh_b) {
+  if ((!body_awake(c, a) && !body_awake(c, b)) || !body_responds(c, a) || !body_responds(c, b)) return false;
+  uint32_t ka = c.w.bodies[a].shape_kind, kb = c.w.bodies[b].shape_kind;
+  if ((ka == k::SHAPE_COMPOUND) == (kb == k::SHAPE_COMPOUND)) return false;
+  uint32_t comp_b = ka == k::SHAPE_COMPOUND ? a : b, other = ka == k::SHAPE_COMPOUND ? b : a;
+  if (c.w.bodies[other].shape_kind != k::SHAPE_MESH) return false;
+  if (!overlaps(child_world_box(c, comp_b), body_box(c, other))) return false;
+  *car = comp_b;
+  *mesh_b = other;
+  return true;
+}
 // P3b (:230-282): the retained pairs in pair-array order. Returns the number of unimplemented pair algorithms reached.
 RS_FN uint32_t phase_p3b(ArenaState& s, const StaticData& w) {
-  Np c = {s, w, 0};
+  Np c = {s, w, 0, nullptr};
   uint32_t n = s.n_pairs < (uint32_t)MAX_PAIRS ? s.n_pairs : (uint32_t)MAX_PAIRS;
+  SatBatch sb;
+  sb.n_jobs = sb.n_tasks = sb.next_job = sb.ok = 0;
+  for (uint32_t i = 0; i < n; ++i) {  // the car-mesh jobs' candidate triangles, in pair order
+    Pair p = s.pairs[i];
+    uint32_t car, mesh_b;
+    if (reaches_car_mesh(c, w.proxies[p.proxy0].client_body, w.proxies[p.proxy1].client_body, &car, &mesh_b))
+      sat_collect(c, car, mesh_b, sb);
+  }
+  sat_batch_run(c, sb);
+  c.sb = &sb;
   for (uint32_t i = 0; i < n; ++i) {
     Pair p = s.pairs[i];
     near_callback(c, w.proxies[p.proxy0].client_body, w.proxies[p.proxy1].client_body);

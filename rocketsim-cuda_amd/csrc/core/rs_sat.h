@@ -35,6 +35,7 @@ RS_FN V3 orient_pen(V3 a, V3 to_box, V3 face, bool has_face) {
 }
 // :525 axis_gap_overlap: the box's radius on axis a against the triangle's projection
 RS_NOINL void gap_overlap(V3 a, const V3 q[3], V3 half, float* gap, float* overlap, float* side) {
+  RS_PRIVATE(q); RS_PRIVATE(gap); RS_PRIVATE(overlap); RS_PRIVATE(side);  // locals of the caller
   float r = (half.x * fabs_(a.x) + half.y * fabs_(a.y)) + half.z * fabs_(a.z);
   float d0 = dot(q[0], a), d1 = dot(q[1], a), d2 = dot(q[2], a);
   float mn = fmin_rs(fmin_rs(d0, d1), d2), mx = fmax_rs(fmax_rs(d0, d1), d2);
@@ -48,6 +49,7 @@ RS_NOINL void gap_overlap(V3 a, const V3 q[3], V3 half, float* gap, float* overl
 }
 // :341 consider_axis: true when the axis rejects the pair
 RS_NOINL bool consider_axis(SatSweep* o, V3 a, SatKind kind, const V3 q[3], V3 half, float margin, float maxd) {
+  RS_PRIVATE(o); RS_PRIVATE(q);  // locals of the caller
   float gap, overlap, side;
   gap_overlap(a, q, half, &gap, &overlap, &side);
   if (gap > 0.0f) {
@@ -96,6 +98,7 @@ RS_FN bool box_face(SatSweep* o, float tmin, float tmax, float h, int axis, floa
 // :359 sat_sweep; false for V3's None (a rejecting axis)
 RS_NOINL bool sat_sweep(V3 half, const V3 q[3], V3 tmin, V3 tmax, const V3 edges[3], V3 face, bool has_face, V3 centroid, float margin,
                         float maxd, SatSweep* o) {
+  RS_PRIVATE(q); RS_PRIVATE(edges); RS_PRIVATE(o);  // locals of the caller
   o->has_sep = false;
   o->best_gap = 0.0f;
   o->best_sep_axis = splat(0.0f);
@@ -133,6 +136,7 @@ RS_FN V3 tri_support(const V3 q[3], V3 dir) {  // :554 first maximum
   return best;
 }
 RS_NOINL int clip_halfspace(const V3* in, int n, V3* out, int axis, float bound, bool keep_above) {  // :607
+  RS_PRIVATE(in); RS_PRIVATE(out);  // locals of the caller
   if (n == 0) return 0;
   int m = 0;
   V3 prev = in[n - 1];
@@ -152,6 +156,7 @@ RS_NOINL int clip_halfspace(const V3* in, int n, V3* out, int axis, float bound,
   return m;
 }
 RS_NOINL bool clipped_centroid(V3 half, const V3 q[3], V3* res) {  // :576
+  RS_PRIVATE(q); RS_PRIVATE(res);  // locals of the caller
   V3 poly[9], tmp[9];
   for (int i = 0; i < 9; ++i) {
     poly[i] = splat(0.0f);
@@ -191,6 +196,7 @@ RS_FN V3 closest_on_degenerate(V3 p, const V3 q[3]) {  // :669
 }
 // segment-segment closest points (:690), with the near-parallel overlap midpoint rule
 RS_NOINL void segseg_closest(V3 p1, V3 q1, V3 p2, V3 q2, V3* o1, V3* o2) {
+  RS_PRIVATE(o1); RS_PRIVATE(o2);  // locals of the caller
   V3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
   float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
   const float eps = 1e-30f;
@@ -278,6 +284,7 @@ RS_FN bool clip_segment_box(V3 a, V3 b, V3 half, V3* o0, V3* o1) {  // :783
   return true;
 }
 RS_NOINL bool pen_witness(V3 half, const V3 q[3], SatKind kind, V3 n, V3* res) {  // :735
+  RS_PRIVATE(q); RS_PRIVATE(res);  // locals of the caller
   if (kind.tag == SAT_TRI_FACE) {
     float pd = dot(n, q[0]);
     V3 corner = box_corner_toward(half, -n);
@@ -321,6 +328,7 @@ RS_NOINL bool pen_witness(V3 half, const V3 q[3], SatKind kind, V3 n, V3* res) {
   return false;
 }
 RS_NOINL V3 closest_on_tri(V3 p, const V3 q[3], V3 ab, V3 ac) {  // :848 (Ericson's region test)
+  RS_PRIVATE(q);  // locals of the caller
   V3 ap = p - q[0];
   float d1 = dot(ab, ap), d2 = dot(ac, ap);
   if (d1 <= 0.0f && d2 <= 0.0f) return q[0];
@@ -342,6 +350,7 @@ RS_NOINL V3 closest_on_tri(V3 p, const V3 q[3], V3 ab, V3 ac) {  // :848 (Ericso
   return q[0] + ab * v + ac * ww;
 }
 RS_NOINL void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri, V3* o_box) {  // :899
+  RS_PRIVATE(o_tri); RS_PRIVATE(o_box);  // locals of the caller
   V3 d2 = b1 - b0, r = tp - b0;
   float e = dot(d2, d2), f = dot(d2, r);
   const float eps = 1e-30f;
@@ -389,6 +398,7 @@ RS_NOINL void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri,
 }
 // :960 the 47 candidate pairs: 3 clamped vertices, 8 box corners against the triangle, 3 x 12 edge pairs
 RS_NOINL void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr) {
+  RS_PRIVATE(q); RS_PRIVATE(edges); RS_PRIVATE(pr);  // locals of the caller
   int n = 0;
   for (int i = 0; i < 3; ++i) {
     V3 pb = clamp_to_box(q[i], half);
@@ -424,6 +434,7 @@ RS_FN SatPair first_min_pair(const SatPair* pr) {  // strict <: the first pair w
 }
 // :1096 flat_tie_segment: the triangle-side endpoints of a flat contact interval, or false
 RS_NOINL bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
+  RS_PRIVATE(pr); RS_PRIVATE(r0); RS_PRIVATE(r1);  // locals of the caller
   float link = 0.5f * max_elem(half);
   if (link <= 0.0f || !ffinite(link)) return false;
   float slope0 = SAT_TIE_EPS / link;
@@ -488,6 +499,7 @@ RS_NOINL bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
 }
 // :52 box_triangle_sat: q is the triangle in the box frame, half the unmargined half extents. True for V3's Some.
 RS_NOINL bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
+  RS_PRIVATE(&box_trans); RS_PRIVATE(q); RS_PRIVATE(out);  // locals of the caller
   V3 tmin = vmin(vmin(q[0], q[1]), q[2]), tmax = vmax(vmax(q[0], q[1]), q[2]);
   for (int ax = 0; ax < 3; ++ax) {  // pre-screen, X then Y then Z
     float lo = comp(tmin, ax), hi = comp(tmax, ax), h = comp(half, ax);

@@ -50,6 +50,17 @@ struct DeltaBank {
     p[2 * blockDim.x] = v.z;
   }
 };
+// Wave-wide staging through the same slices (the narrowphase's batched separating-axis tests, between the BVH walks): slot
+// r of a wave is the slice of the wave's lane r, so any lane of the wave can write or read any slot. LDS accesses of one
+// wave execute in order; the fences only keep the compiler from moving them across the hand-off points.
+__device__ inline float* wave_slot(uint32_t slot, uint32_t k) {
+  return reinterpret_cast<float*>(rs_trav_lds) + (k * blockDim.x + (threadIdx.x & ~63u) + slot);
+}
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #else
 struct DeltaBank {
   V3 v[BANK_BODIES][4];

@@ -319,7 +319,9 @@ constexpr int NUM_PHASES = 10;
 enum PhaseStatus : int { PHASE_OK = 0, PHASE_UNIMPLEMENTED = -1, PHASE_BAD_INDEX = -2 };
 
 // Phase k of one tick (0 P0, 1 P1, 2 P2, 3 P3a, 4 P3b, 5 P4, 6 P5, 7 P6, 8 P7, 9 P8): state at boundary k to boundary k + 1.
-RS_NOINL int run_phase(int kk, ArenaState& s, const StaticData& w) {
+RS_NOINL int run_phase_in(int kk, RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp) {
+  ArenaState& s = *(ArenaState*)sp;
+  const StaticData& w = *(const StaticData*)wp;
   switch (kk) {
     case 0: phase_p0(s, w); return PHASE_OK;
     case 1: phase_p1(s, w); return PHASE_OK;
@@ -333,6 +335,10 @@ RS_NOINL int run_phase(int kk, ArenaState& s, const StaticData& w) {
     case 9: phase_p8(s, w); return PHASE_OK;
     default: return PHASE_BAD_INDEX;
   }
+}
+// Every caller holds its arena in a lane-private copy and the StaticData in global memory.
+RS_FN int run_phase(int kk, ArenaState& s, const StaticData& w) {
+  return run_phase_in(kk, (RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w);
 }
 // One tick (base.rs step_tick): the ten phases in order.
 RS_FN int step_tick(ArenaState& s, const StaticData& w) {
