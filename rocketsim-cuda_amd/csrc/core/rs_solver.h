@@ -241,14 +241,16 @@ RS_FN Row friction_row(int ia, int ib, const SolverBody& sa, const SolverBody& s
 }
 // The row solves read and write the bodies' deltas through the bank (fields: 0 dlin, 1 dang, 2 push, 3 turn); the
 // read-only body constants (inverse mass, whether it is a dynamic body) come from the SolverBody.
-enum BankField { BF_DLIN = 0, BF_DANG = 1, BF_PUSH = 2, BF_TURN = 3 };
-RS_FN float resolve_split(Row& r, const SolverBody& a, const SolverBody& b, DeltaBank& bank) {  // :518-565
+enum BankField { BF_DLIN = 0, BF_DANG = 1, BF_PUSH = 2, BF_TURN = 3, BF_INV_MASS = 4 };
+// The row solves read the bodies' inverse masses and kinds from the bank (a body that is not dynamic takes no delta).
+RS_FN float resolve_split(Row& r, DeltaBank& bank) {  // :518-565
   if (r.rhs_pen == 0.0f) return 0.0f;
+  const bool ha = bank.dynamic(r.a), hb = bank.dynamic(r.b);
   float di = r.rhs_pen;
-  V3 pa = a.has_orig ? bank.get(r.a, BF_PUSH) : splat(0.0f), ta = a.has_orig ? bank.get(r.a, BF_TURN) : splat(0.0f);
-  V3 pb = b.has_orig ? bank.get(r.b, BF_PUSH) : splat(0.0f), tb = b.has_orig ? bank.get(r.b, BF_TURN) : splat(0.0f);
-  float d1 = a.has_orig ? bullet_dot(r.n1, pa) + bullet_dot(r.r1xn, ta) : 0.0f;
-  float d2 = b.has_orig ? bullet_dot(r.n2, pb) + bullet_dot(r.r2xn, tb) : 0.0f;
+  V3 pa = ha ? bank.get(r.a, BF_PUSH) : splat(0.0f), ta = ha ? bank.get(r.a, BF_TURN) : splat(0.0f);
+  V3 pb = hb ? bank.get(r.b, BF_PUSH) : splat(0.0f), tb = hb ? bank.get(r.b, BF_TURN) : splat(0.0f);
+  float d1 = ha ? bullet_dot(r.n1, pa) + bullet_dot(r.r1xn, ta) : 0.0f;
+  float d2 = hb ? bullet_dot(r.n2, pb) + bullet_dot(r.r2xn, tb) : 0.0f;
   di = di - d1 * r.jinv;
   di = di - d2 * r.jinv;
   float sum = r.applied_push + di;
@@ -258,22 +260,23 @@ RS_FN float resolve_split(Row& r, const SolverBody& a, const SolverBody& b, Delt
   } else {
     r.applied_push = sum;
   }
-  if (a.has_orig) {
-    bank.set(r.a, BF_PUSH, pa + r.n1 * a.inv_mass * di);
+  if (ha) {
+    bank.set(r.a, BF_PUSH, pa + r.n1 * bank.get(r.a, BF_INV_MASS) * di);
     bank.set(r.a, BF_TURN, ta + r.ang_a * di);
   }
-  if (b.has_orig) {  // re-read: a and b may be the same body
-    bank.set(r.b, BF_PUSH, bank.get(r.b, BF_PUSH) + r.n2 * b.inv_mass * di);
+  if (hb) {  // re-read: a and b may be the same body
+    bank.set(r.b, BF_PUSH, bank.get(r.b, BF_PUSH) + r.n2 * bank.get(r.b, BF_INV_MASS) * di);
     bank.set(r.b, BF_TURN, bank.get(r.b, BF_TURN) + r.ang_b * di);
   }
   return di / r.jinv;
 }
-RS_FN float resolve_row(Row& r, const SolverBody& a, const SolverBody& b, DeltaBank& bank, bool two_sided) {  // :427-516
+RS_FN float resolve_row(Row& r, DeltaBank& bank, bool two_sided) {  // :427-516
+  const bool ha = bank.dynamic(r.a), hb = bank.dynamic(r.b);
   float di = r.rhs;
-  V3 la = a.has_orig ? bank.get(r.a, BF_DLIN) : splat(0.0f), aa = a.has_orig ? bank.get(r.a, BF_DANG) : splat(0.0f);
-  V3 lb = b.has_orig ? bank.get(r.b, BF_DLIN) : splat(0.0f), ab = b.has_orig ? bank.get(r.b, BF_DANG) : splat(0.0f);
-  float d1 = a.has_orig ? bullet_dot(r.n1, la) + bullet_dot(r.r1xn, aa) : 0.0f;
-  float d2 = b.has_orig ? bullet_dot(r.n2, lb) + bullet_dot(r.r2xn, ab) : 0.0f;
+  V3 la = ha ? bank.get(r.a, BF_DLIN) : splat(0.0f), aa = ha ? bank.get(r.a, BF_DANG) : splat(0.0f);
+  V3 lb = hb ? bank.get(r.b, BF_DLIN) : splat(0.0f), ab = hb ? bank.get(r.b, BF_DANG) : splat(0.0f);
+  float d1 = ha ? bullet_dot(r.n1, la) + bullet_dot(r.r1xn, aa) : 0.0f;
+  float d2 = hb ? bullet_dot(r.n2, lb) + bullet_dot(r.r2xn, ab) : 0.0f;
   di = di - d1 * r.jinv;
   di = di - d2 * r.jinv;
   float sum = r.applied + di;
@@ -286,12 +289,12 @@ RS_FN float resolve_row(Row& r, const SolverBody& a, const SolverBody& b, DeltaB
   } else {
     r.applied = sum;
   }
-  if (a.has_orig) {
-    bank.set(r.a, BF_DLIN, la + r.n1 * a.inv_mass * di);
+  if (ha) {
+    bank.set(r.a, BF_DLIN, la + r.n1 * bank.get(r.a, BF_INV_MASS) * di);
     bank.set(r.a, BF_DANG, aa + r.ang_a * di);
   }
-  if (b.has_orig) {  // re-read: a and b may be the same body
-    bank.set(r.b, BF_DLIN, bank.get(r.b, BF_DLIN) + r.n2 * b.inv_mass * di);
+  if (hb) {  // re-read: a and b may be the same body
+    bank.set(r.b, BF_DLIN, bank.get(r.b, BF_DLIN) + r.n2 * bank.get(r.b, BF_INV_MASS) * di);
     bank.set(r.b, BF_DANG, bank.get(r.b, BF_DANG) + r.ang_b * di);
   }
   return di / r.jinv;
@@ -419,44 +422,60 @@ RS_FN void solver_setup(Solver& sv, ArenaState& s, const StaticData& w, float dt
     push_special(sv, s, w, acc, dt);
   }
 }
+// The iterations (:196-251). Each loop reads its next row while it solves the current one (a row's solve never writes
+// another row), keeps the rows still running in a bit mask, and takes the contact impulses a friction row needs from the
+// bank; what a solve changes in a row is written back to it.
 RS_FN void solver_iterate(Solver& sv) {  // :196-251
+  static_assert(MAX_ROWS <= BANK_ROWS && MAX_SOLVER_BODIES <= BANK_BODIES, "the bank holds every row and body");
+  static_assert(MAX_ROWS <= 32, "the running rows are a 32-bit mask");
   DeltaBank bank;
   for (int i = 0; i < sv.n_bodies && i < BANK_BODIES; ++i) {
     bank.set(i, BF_DLIN, sv.bodies[i].dlin);
     bank.set(i, BF_DANG, sv.bodies[i].dang);
     bank.set(i, BF_PUSH, sv.bodies[i].push);
     bank.set(i, BF_TURN, sv.bodies[i].turn);
+    bank.set(i, BF_INV_MASS, sv.bodies[i].inv_mass);
+    bank.set_dynamic(i, sv.bodies[i].has_orig != 0);
   }
-  bool run[MAX_ROWS];
-  int remaining = sv.n_contact;
-  for (int i = 0; i < sv.n_contact; ++i) run[i] = true;
-  for (int it = 0; it < SOLVER_ITERATIONS && remaining != 0; ++it) {
-    for (int i = 0; i < sv.n_contact; ++i) {
-      if (!run[i]) continue;
-      Row& r = sv.contact[i];
-      float res = resolve_split(r, sv.bodies[r.a], sv.bodies[r.b], bank);
-      if (res * res == 0.0f) {
-        run[i] = false;
-        --remaining;
-      }
+  const int nc = sv.n_contact, nf = sv.n_friction;
+  for (int i = 0; i < nc; ++i) bank.set_applied(i, sv.contact[i].applied);
+  uint32_t run = nc >= 32 ? ~0u : (1u << nc) - 1u;
+  for (int it = 0; it < SOLVER_ITERATIONS && run != 0; ++it) {
+    Row next = nc > 0 ? sv.contact[0] : Row{};
+    for (int i = 0; i < nc; ++i) {
+      Row r = next;
+      if (i + 1 < nc) next = sv.contact[i + 1];
+      if (!((run >> i) & 1u)) continue;
+      float res = resolve_split(r, bank);
+      sv.contact[i].applied_push = r.applied_push;
+      if (res * res == 0.0f) run &= ~(1u << i);
     }
   }
   for (int it = 0; it < SOLVER_ITERATIONS; ++it) {
     float lsr = 0.0f;
-    for (int i = 0; i < sv.n_contact; ++i) {
-      Row& r = sv.contact[i];
+    Row next = nc > 0 ? sv.contact[0] : Row{};
+    for (int i = 0; i < nc; ++i) {
+      Row r = next;
+      if (i + 1 < nc) next = sv.contact[i + 1];
       if (r.split_only) continue;
-      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], bank, false);
+      float res = resolve_row(r, bank, false);
+      sv.contact[i].applied = r.applied;
+      bank.set_applied(i, r.applied);
       lsr = fmax_rs(res * res, lsr);
     }
-    for (int i = 0; i < sv.n_friction; ++i) {
-      Row& r = sv.friction[i];
-      float total = sv.contact[r.friction_idx].applied;
+    next = nf > 0 ? sv.friction[0] : Row{};
+    for (int i = 0; i < nf; ++i) {
+      Row r = next;
+      if (i + 1 < nf) next = sv.friction[i + 1];
+      float total = bank.applied(r.friction_idx);
       if (total <= 0.0f) continue;
       float lim = r.friction * total;
       r.lower = fneg(lim);
       r.upper = lim;
-      float res = resolve_row(r, sv.bodies[r.a], sv.bodies[r.b], bank, true);
+      float res = resolve_row(r, bank, true);
+      sv.friction[i].lower = r.lower;
+      sv.friction[i].upper = r.upper;
+      sv.friction[i].applied = r.applied;
       lsr = fmax_rs(res * res, lsr);
     }
     if (lsr == 0.0f) break;

@@ -12,9 +12,12 @@ namespace rs {
 
 constexpr int TRAV_CAP = 128;
 constexpr int TRAV_LDS = 32;
-// The solver's delta bank (below): the mutable velocity and split-impulse deltas of up to 8 solver bodies.
+// The solver's bank (below): per solver body (up to 8) the mutable velocity and split-impulse deltas, its inverse mass and
+// whether it is a dynamic body (16 words); then the contact rows' applied impulses (up to 32), which the friction rows read.
 constexpr int BANK_BODIES = 8;
-constexpr int BANK_WORDS = BANK_BODIES * 12;
+constexpr int BANK_BODY_WORDS = 16;
+constexpr int BANK_ROWS = 32;
+constexpr int BANK_WORDS = BANK_BODIES * BANK_BODY_WORDS + BANK_ROWS;
 // Both live in the same per-lane LDS slice (they are used in different phases), entry-major: word k of lane t of the
 // work-group is at k * blockDim + t.
 constexpr int LANE_LDS_WORDS = TRAV_LDS > BANK_WORDS ? TRAV_LDS : BANK_WORDS;
@@ -36,19 +39,25 @@ struct TravStack {
     return n < TRAV_LDS ? rs_trav_lds[(uint32_t)n * blockDim.x + threadIdx.x] : spill[n - TRAV_LDS];
   }
 };
-// The solver bodies' deltas (dlin, dang, push, turn) during the iterations: in LDS on the GPU, so that a row reading a
-// body the previous row just updated waits on LDS rather than on a private-memory round trip.
+// The solver's bank during the iterations: in LDS on the GPU, so that a row reading a body the previous row just updated,
+// or the impulse of the contact row a friction row is tied to, waits on LDS rather than on a private-memory round trip.
+// Fields 0-4 of a body are the deltas (dlin, dang, push, turn) and the inverse mass; word 15 is 1 for a dynamic body.
 struct DeltaBank {
+  __device__ static float* at(uint32_t word) { return reinterpret_cast<float*>(rs_trav_lds) + (word * blockDim.x + threadIdx.x); }
   __device__ V3 get(int body, int field) const {
-    const float* p = reinterpret_cast<const float*>(rs_trav_lds) + ((uint32_t)(body * 12 + field * 3) * blockDim.x + threadIdx.x);
+    const float* p = at((uint32_t)(body * BANK_BODY_WORDS + field * 3));
     return V3{p[0], p[blockDim.x], p[2 * blockDim.x]};
   }
   __device__ void set(int body, int field, V3 v) {
-    float* p = reinterpret_cast<float*>(rs_trav_lds) + ((uint32_t)(body * 12 + field * 3) * blockDim.x + threadIdx.x);
+    float* p = at((uint32_t)(body * BANK_BODY_WORDS + field * 3));
     p[0] = v.x;
     p[blockDim.x] = v.y;
     p[2 * blockDim.x] = v.z;
   }
+  __device__ bool dynamic(int body) const { return *at((uint32_t)(body * BANK_BODY_WORDS + 15)) != 0.0f; }
+  __device__ void set_dynamic(int body, bool d) { *at((uint32_t)(body * BANK_BODY_WORDS + 15)) = d ? 1.0f : 0.0f; }
+  __device__ float applied(int row) const { return *at((uint32_t)(BANK_BODIES * BANK_BODY_WORDS + row)); }
+  __device__ void set_applied(int row, float v) { *at((uint32_t)(BANK_BODIES * BANK_BODY_WORDS + row)) = v; }
 };
 // Wave-wide staging through the same slices (the narrowphase's batched separating-axis tests, between the BVH walks): slot
 // r of a wave is the slice of the wave's lane r, so any lane of the wave can write or read any slot. LDS accesses of one
@@ -63,9 +72,15 @@ __device__ inline void wave_lds_sync() {
 }
 #else
 struct DeltaBank {
-  V3 v[BANK_BODIES][4];
+  V3 v[BANK_BODIES][5];
+  bool dyn[BANK_BODIES];
+  float row_applied[BANK_ROWS];
   V3 get(int body, int field) const { return v[body][field]; }
   void set(int body, int field, V3 x) { v[body][field] = x; }
+  bool dynamic(int body) const { return dyn[body]; }
+  void set_dynamic(int body, bool d) { dyn[body] = d; }
+  float applied(int row) const { return row_applied[row]; }
+  void set_applied(int row, float x) { row_applied[row] = x; }
 };
 struct TravStack {
   uint32_t a[TRAV_CAP];
