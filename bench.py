@@ -156,6 +156,99 @@ def compare_orders(rsim, dev, dw, model, states_host, controls, block, plain=Non
     return out
 
 
+def open_play_leg(rsim, dev, decisions, block, threads):
+    """16,384 open-play arenas over the seven hitbox-preset pairings (preset i vs preset i, arena a in pairing a mod 7),
+    `decisions` plain per-launch decisions with per-arena worlds; every result word and final state compared with the
+    CPU build of the same decisions."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    n = 16384
+    worlds = np.stack([rsim.build_world(p, p) for p in range(7)])
+    variant = (np.arange(n) % 7).astype(np.uint32)
+    states = rsim.sample_arenas_in(worlds, variant, rsim.SCEN_OPEN, 2031)
+    controls = [rsim.sample_controls(4646, d, n) for d in range(decisions)]
+    d_w = dev.upload_worlds(worlds)
+    d_v = dev.upload_words(variant)
+    d_s = dev.upload_states(states)
+    d_c = [dev.upload_controls(c) for c in controls]
+    d_r = [dev.alloc_words(n) for _ in range(decisions)]
+    dev.sync()
+    dev.timer_begin()
+    for k in range(decisions):
+        dev.sim8_variants(d_s, d_w, d_v, d_c[k], d_r[k], block)
+    ms = dev.timer_end()
+    res = [dev.read_words(r) for r in d_r]
+    fin = dev.download_states(d_s)
+    o = orc.Oracle()
+    cpu = states.copy()
+    diffs = 0
+    t0 = time.perf_counter()
+    for k in range(decisions):
+        diffs += int((o.sim8_in(cpu, worlds, variant, controls[k], threads=threads) != res[k]).sum())
+    cpu_s = time.perf_counter() - t0
+    diffs += int((cpu[:, :2567] != fin[:, :2567]).sum())
+    stepped = stepped_ticks(res)
+    return {"arenas": n, "decisions": decisions, "worlds": 7, "arena_ticks_per_s": stepped / (ms / 1000.0),
+            "ms_per_decision": ms / decisions, "kernel": "k_sim8 (per-arena worlds)", "word_diffs_vs_cpu": diffs,
+            "compared": "every arena, every decision", "cpu_arena_ticks_per_s": stepped / cpu_s, "cpu_threads": threads}
+
+
+def scale_leg(rsim, dev, decisions, block, threads, chunk=50):
+    """131,072 arenas of the 1:1:1 kickoff / open play / contact mix, each in its own world of a table of 7 hitbox presets
+    x 16 mutator draws (gravity z U(-800, -500), boost acceleration x U(0.8, 1.2), ball radius U(91.25, 100)); `decisions`
+    decisions as persistent launches of `chunk` decisions each (k_run), all captured in one HIP graph together with a
+    gather of a 1% sample of the arenas after every chunk; the sample's result words and states are compared with the CPU
+    build after the run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    n = 131072
+    rng = np.random.default_rng(2032)
+    worlds = np.stack([rsim.build_world(p, p, float(rng.uniform(-800, -500)), float(rng.uniform(0.8, 1.2)),
+                                        float(rng.uniform(91.25, 100))) for p in range(7) for _ in range(16)])
+    variant = rng.integers(0, worlds.shape[0], n).astype(np.uint32)
+    states = rsim.sample_arenas_in(worlds, variant, rsim.SCEN_MIXED, 2033)
+    sample = np.sort(rng.choice(n, n // 100, replace=False)).astype(np.uint32)
+    d_w = dev.upload_worlds(worlds)
+    d_v = dev.upload_words(variant)
+    d_s = dev.upload_states(states)
+    d_sample = dev.upload_words(sample)
+    chunks = decisions // chunk
+    d_c, d_r, d_g, ctl_sample = [], [], [], []
+    for c in range(chunks):
+        ctl = np.concatenate([rsim.sample_controls(4747, c * chunk + k, n) for k in range(chunk)])
+        d_c.append(dev.upload_words(ctl))
+        ctl_sample.append(ctl.reshape(chunk, n, 16)[:, sample].copy())
+        d_r.append(dev.alloc_words(chunk * n))
+        d_g.append(dev.alloc_states(sample.size))
+    dev.sync()
+    dev.capture_begin()
+    for c in range(chunks):
+        dev.run(d_s, d_w, d_c[c], d_r[c], chunk, variant=d_v, block=block)
+        rsim._check(rsim.lib().rsim_gather_states(dev.handle, d_s.handle, d_sample.handle, d_g[c].handle), "rsim_gather_states")
+    graph = dev.capture_end()
+    dev.timer_begin()
+    graph.launch()
+    ms = dev.timer_end()
+    graph.free()
+    stepped = 0
+    o = orc.Oracle()
+    cpu = np.ascontiguousarray(states[sample])
+    vs = variant[sample]
+    diffs = 0
+    for c in range(chunks):
+        r = dev.read_words(d_r[c]).reshape(chunk, n)
+        stepped += stepped_ticks(list(r))
+        for k in range(chunk):
+            diffs += int((o.sim8_in(cpu, worlds, vs, np.ascontiguousarray(ctl_sample[c][k]), threads=threads) != r[k, sample]).sum())
+        diffs += int((cpu[:, :2567] != dev.download_states(d_g[c])[:, :2567]).sum())
+    return {"arenas": n, "decisions": chunks * chunk, "worlds": int(worlds.shape[0]), "arena_ticks_per_s": stepped / (ms / 1000.0),
+            "ms_per_decision": ms / (chunks * chunk), "kernel": "k_run, %d decisions per launch, one HIP graph" % chunk,
+            "word_diffs_vs_cpu": diffs, "compared": "a 1%% sample (%d arenas): result words of every decision, states every %d"
+            % (sample.size, chunk), "state_gigabytes": n * 11136 / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +259,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=ARENAS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cost", action="store_true", help="skip the learned cost-model legs")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 16,384-arena open-play and 131,072-arena scale legs")
+    ap.add_argument("--scale-decisions", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this stepper runs on one GPU; --gpus %d is not supported (arenas are independent, so N GPUs would "
@@ -268,6 +363,13 @@ def main():
             rsim, dev, dw, model, rsim.sample_arenas(world, rsim.SCEN_MIXED, 2027, 2 * n),
             [rsim.sample_controls(4444, d, 2 * n) for d in range(k2)], args.block)
 
+    configs = None
+    if not args.no_configs:
+        print("[bench] 16,384 open-play arenas, 7 preset pairings, 100 decisions", file=sys.stderr, flush=True)
+        configs = {"open16384": open_play_leg(rsim, dev, 100, args.block, host_threads())}
+        print("[bench] 131,072 arenas, %d worlds, %d decisions in one graph" % (112, args.scale_decisions), file=sys.stderr, flush=True)
+        configs["scale131072"] = scale_leg(rsim, dev, args.scale_decisions, args.block, host_threads())
+
     cpu, diffs = None, None
     if not args.no_cpu:
         print("[bench] CPU baseline on %d arenas" % min(args.cpu_sample, n), file=sys.stderr, flush=True)
@@ -300,6 +402,7 @@ def main():
         "nominal_arena_ticks_per_s": nominal / wall,
         "flagged_arenas": flags,
         "cost_regrouped": cost,
+        "configs": configs,
         "multi_decision_launch": {"arena_ticks_per_s": stepped / run_wall, "kernel_ms": run_ms, "kernel": "k_run",
                                   "identical_to_plain": run_same},
         "graph_replay": {"arena_ticks_per_s": stepped / (graph_ms / 1000.0), "ms": graph_ms, "identical_to_plain": graph_same},

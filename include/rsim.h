@@ -155,7 +155,9 @@ uint64_t rsim_cost_order_work_words(uint32_t n);
 int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* work, rsim_words* perm);
 /* Uploads a host permutation of 0..n-1 (checked) for the regrouping calls. */
 int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsim_words* dst);
-/* dst arena i = src arena perm[i] (whole AoSoA-64 states; src and dst distinct, same n). */
+/* dst arena i = src arena perm[i] for the dst->n arenas of dst (whole AoSoA-64 states; src and dst distinct; perm holds
+ * at least dst->n indices; an index past src leaves that arena of dst as it was). With dst->n == src->n and a permutation
+ * this is a reordering; with a smaller dst, a sample. */
 int rsim_gather_states(rsim_device* dev, const rsim_states* src, const rsim_words* perm, rsim_states* dst);
 /* Items of words_per_item words: gather (dst[i] = src[perm[i]]) or, with scatter != 0, dst[perm[i]] = src[i]. */
 int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words* perm, uint32_t n, uint32_t words_per_item,
@@ -201,6 +203,10 @@ const char* rsim_last_error(void);
 int rsim_build_world(int preset0, int preset1, float gravity_z, float boost_accel_scale, float ball_radius, void* world_out);
 /* Arenas first..first+n-1 of a seeded workload: arena i depends only on (seed, i), so any subrange is reproducible. */
 int rsim_sample_arenas(const void* world, int scenario, uint64_t seed, uint32_t first, uint32_t n, void* states_out);
+/* As rsim_sample_arenas, with arena first + i drawn in worlds[variant[i]] (a host table of `count` StaticData, e.g. the
+ * one passed to rsim_upload_worlds); arena i is the same draw rsim_sample_arenas makes in that world. */
+int rsim_sample_arenas_in(const void* worlds, uint32_t count, const uint32_t* variant, int scenario, uint64_t seed,
+                          uint32_t first, uint32_t n, void* states_out);
 /* The controls of one decision for arenas first..first+n-1 (Controls[n][2]): axes uniform in {-1, 0, 1}, jump / boost /
  * handbrake Bernoulli(0.3). */
 int rsim_sample_controls(uint64_t seed, uint32_t decision, uint32_t first, uint32_t n, void* controls_out);

@@ -42,6 +42,23 @@ void rsim_cpu_sim8_batch(ArenaState* states, const StaticData* w, const Controls
   }
   for (auto& th : pool) th.join();
 }
+// rsim_cpu_sim8_batch with arena a stepped in worlds[variant[a]] (the CPU side of rsim_sim8_variants / rsim_run with a
+// variant table); an index past the table leaves the arena untouched with RSIM_SIM8_BAD_VARIANT, as on the device.
+void rsim_cpu_sim8_batch_in(ArenaState* states, const StaticData* worlds, uint32_t count, const uint32_t* variant,
+                            const Controls* controls, uint32_t* result, uint32_t n, uint32_t threads) {
+  if (threads < 1) threads = 1;
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < threads; ++t) {
+    uint32_t lo = (uint32_t)((uint64_t)n * t / threads), hi = (uint32_t)((uint64_t)n * (t + 1) / threads);
+    pool.emplace_back([=] {
+      for (uint32_t a = lo; a < hi; ++a)
+        result[a] = variant[a] < count ? run_decision(states[a], worlds[variant[a]], controls + 2 * (size_t)a, TICKS_PER_DECISION,
+                                                      NoTrace{})
+                                       : 0x20000000u;
+    });
+  }
+  for (auto& th : pool) th.join();
+}
 // Word offset of cars[car].controls in ArenaState (for setting controls before a per-phase run).
 uint64_t rsim_cpu_controls_word(int car) { return (offsetof(ArenaState, cars) + car * sizeof(Car) + offsetof(Car, controls)) / 4; }
 // Brute force: move_cells against the remove-then-push sequence it replaces, on `iters` random sorted lists (lengths up

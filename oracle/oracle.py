@@ -46,6 +46,7 @@ class Oracle:
         L.rsim_cpu_step_ticks.argtypes = [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32), _vp]
         L.rsim_cpu_sim8_aos.argtypes = [_vp, _vp, _vp, _vp, _u32]
         L.rsim_cpu_sim8_batch.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32]
+        L.rsim_cpu_sim8_batch_in.argtypes = [_vp, _vp, _u32, _vp, _vp, _vp, _u32, _u32]
         L.rsim_cpu_libm_unary.argtypes = [ctypes.c_int, _u64, _u64, _vp]
         L.rsim_cpu_libm_unary_list.argtypes = [ctypes.c_int, _vp, _u64, _vp]
         L.rsim_cpu_libm_binary.argtypes = [ctypes.c_int, _vp, _vp, _u64, _vp]
@@ -85,6 +86,17 @@ class Oracle:
         res = np.zeros(max(n, 1), np.uint32)
         assert states.flags.c_contiguous and controls.flags.c_contiguous
         self.L.rsim_cpu_sim8_batch(_p(states), _p(world), _p(controls), _p(res), n, threads)
+        return res[:n]
+
+    def sim8_in(self, states, worlds, variant, controls, threads=1):
+        """sim8 with arena a in worlds[variant[a]] (worlds: uint32[W, static words]); returns the result words."""
+        n = states.shape[0]
+        res = np.zeros(max(n, 1), np.uint32)
+        worlds = np.ascontiguousarray(worlds, np.uint32)
+        variant = np.ascontiguousarray(variant, np.uint32)
+        assert states.flags.c_contiguous and controls.flags.c_contiguous and variant.size >= n
+        self.L.rsim_cpu_sim8_batch_in(_p(states), _p(worlds), worlds.reshape(len(worlds), -1).shape[0], _p(variant), _p(controls),
+                                      _p(res), n, threads)
         return res[:n]
 
     def check_move_cells(self, world, seed, iters):

@@ -26,7 +26,8 @@ hipError_t launch_mlp_train(hipStream_t st, const float* params, const float* fe
 hipError_t launch_adam(hipStream_t st, float* params, float* m, float* v, const float* grad, float lr, float bc1, float bc2);
 hipError_t launch_gather_words(hipStream_t st, const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint32_t n, uint32_t words,
                                bool scatter);
-hipError_t launch_gather_states(hipStream_t st, const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint32_t n);
+hipError_t launch_gather_states(hipStream_t st, const uint32_t* src, uint32_t n_src, const uint32_t* perm, uint32_t* dst,
+                                uint32_t n);
 
 }  // namespace dev
 }  // namespace rs

@@ -240,16 +240,16 @@ __global__ void __launch_bounds__(256) k_scatter_words(const uint32_t* __restric
   uint32_t j = perm[i];
   if (j < n) dst[(size_t)j * words + w] = src[e];
 }
-// AoSoA-64 states: arena i of dst = arena perm[i] of src, the persistent words (the tail is zero at a tick boundary). The
+// AoSoA-64 states: arena i of dst (n arenas) = arena perm[i] of src (n_src arenas), the persistent words (the tail is zero at a tick boundary). The
 // grid's y dimension walks the words, so all arenas of one word are copied together and the source lines they share
 // are read from HBM once.
-__global__ void __launch_bounds__(256) k_gather_states(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm,
-                                                       uint32_t* __restrict__ dst, uint32_t n) {
+__global__ void __launch_bounds__(256) k_gather_states(const uint32_t* __restrict__ src, uint32_t n_src,
+                                                       const uint32_t* __restrict__ perm, uint32_t* __restrict__ dst, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t w = blockIdx.y;
   if (i >= n || w >= (uint32_t)ARENA_WORDS) return;
   uint32_t a = perm[i];
-  if (a < n) dst[aosoa_index(i, w)] = src[aosoa_index(a, w)];
+  if (a < n_src) dst[aosoa_index(i, w)] = src[aosoa_index(a, w)];
 }
 
 // ---- the order by predicted cost ---------------------------------------------------------------------------------------------
@@ -369,8 +369,9 @@ hipError_t launch_gather_words(hipStream_t st, const uint32_t* src, const uint32
   else k_gather_words<<<blocks_of((size_t)n * words, 256), 256, 0, st>>>(src, perm, dst, n, words);
   return hipGetLastError();
 }
-hipError_t launch_gather_states(hipStream_t st, const uint32_t* src, const uint32_t* perm, uint32_t* dst, uint32_t n) {
-  k_gather_states<<<dim3(blocks_of(n, 256), ARENA_WORDS), 256, 0, st>>>(src, perm, dst, n);
+hipError_t launch_gather_states(hipStream_t st, const uint32_t* src, uint32_t n_src, const uint32_t* perm, uint32_t* dst,
+                                uint32_t n) {
+  k_gather_states<<<dim3(blocks_of(n, 256), ARENA_WORDS), 256, 0, st>>>(src, n_src, perm, dst, n);
   return hipGetLastError();
 }
 

@@ -491,10 +491,10 @@ int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsi
 int rsim_gather_states(rsim_device* dev, const rsim_states* src, const rsim_words* perm, rsim_states* dst) {
   int rc = use(dev);
   if (rc) return rc;
-  if (!src || !perm || !dst || src->n != dst->n || src == dst) return fail(RSIM_ERR_INVALID, "rsim_gather_states: bad buffers");
-  if (perm->n < src->n) return fail(RSIM_ERR_SHORT, "rsim_gather_states: permutation too short");
-  if (!src->n) return RSIM_OK;
-  RSIM_HIP(rs::dev::launch_gather_states(dev->stream, src->d, perm->d, dst->d, src->n), "gather states");
+  if (!src || !perm || !dst || src == dst) return fail(RSIM_ERR_INVALID, "rsim_gather_states: bad buffers");
+  if (perm->n < dst->n) return fail(RSIM_ERR_SHORT, "rsim_gather_states: index list too short");
+  if (!dst->n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_gather_states(dev->stream, src->d, src->n, perm->d, dst->d, dst->n), "gather states");
   return RSIM_OK;
 }
 uint64_t rsim_cost_order_work_words(uint32_t n) { return rs::dev::order_work_words(n); }

@@ -921,6 +921,21 @@ int rsim_sample_arenas(const void* world, int scenario, uint64_t seed, uint32_t 
   return RSIM_OK;
 }
 
+int rsim_sample_arenas_in(const void* worlds, uint32_t count, const uint32_t* variant, int scenario, uint64_t seed,
+                          uint32_t first, uint32_t n, void* states_out) {
+  if (!worlds || !variant || !states_out || scenario < 0 || scenario > RSIM_SCEN_OPEN_CONTACT) return RSIM_ERR_INVALID;
+  for (uint32_t i = 0; i < n; ++i)
+    if (variant[i] >= count) return RSIM_ERR_INVALID;
+  const rs::StaticData* w = (const rs::StaticData*)worlds;
+  rs::ArenaState* out = (rs::ArenaState*)states_out;
+  for (uint32_t i = 0; i < n; ++i) {  // the same draws as rsim_sample_arenas, each in its own world
+    rs::world::Rng r{arena_seed(seed, first + i)};
+    int kind = scenario_of(scenario, r.next());
+    rs::world::sample_arena(&out[i], w[variant[i]], kind, r);
+  }
+  return RSIM_OK;
+}
+
 int rsim_sample_controls(uint64_t seed, uint32_t decision, uint32_t first, uint32_t n, void* controls_out) {
   if (!controls_out) return RSIM_ERR_INVALID;
   rs::Controls* out = (rs::Controls*)controls_out;

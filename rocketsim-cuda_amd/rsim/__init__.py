@@ -95,6 +95,7 @@ def lib():
             "rsim_build_world": ([ctypes.c_int, ctypes.c_int, f32, f32, f32, _vp], ctypes.c_int),
             "rsim_sample_arenas": ([_vp, ctypes.c_int, u64, u32, u32, _vp], ctypes.c_int),
             "rsim_sample_controls": ([u64, u32, u32, u32, _vp], ctypes.c_int),
+            "rsim_sample_arenas_in": ([_vp, u32, _vp, ctypes.c_int, u64, u32, u32, _vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -159,6 +160,16 @@ def sample_arenas(world, scenario, seed, n, first=0):
     s = np.zeros((n, ARENA_WORDS), np.uint32)
     _check(lib().rsim_sample_arenas(_ptr(_words(world, STATIC_WORDS, "world")), scenario, seed, first, n, _ptr(s)),
            "rsim_sample_arenas")
+    return s
+
+
+def sample_arenas_in(worlds, variant, scenario, seed, first=0):
+    """ArenaStates of arenas first..first+len(variant)-1, arena i drawn in worlds[variant[i]] (worlds: uint32[W, STATIC_WORDS])."""
+    worlds = np.ascontiguousarray(worlds, np.uint32).reshape(-1, STATIC_WORDS)
+    variant = np.ascontiguousarray(variant, np.uint32)
+    s = np.zeros((variant.size, ARENA_WORDS), np.uint32)
+    _check(lib().rsim_sample_arenas_in(_ptr(worlds), worlds.shape[0], _ptr(variant), scenario, seed, first, variant.size, _ptr(s)),
+           "rsim_sample_arenas_in")
     return s
 
 

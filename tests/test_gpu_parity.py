@@ -318,3 +318,31 @@ def test_run_equals_consecutive_sim8(rsim, oracle, world, device, dev_world):
         assert np.array_equal(got_r[k], gr[k]), "decision %d" % k
     cs, cr = cpu_decisions(oracle, world, states, ctl)
     assert_same(got, list(got_r), cs, cr, "rsim_run")
+
+
+def test_run_per_arena_worlds_and_sample_gather(rsim, oracle, device):
+    """rsim_run with a per-arena world table (presets and mutators) equals the oracle in each arena's world, and a gather
+    of a sample of arenas into a smaller buffer returns exactly those arenas."""
+    note("rsim_run 4 decisions, 2000 arenas in 12 worlds; gather of a 37-arena sample")
+    rng = np.random.default_rng(111)
+    worlds = np.stack([rsim.build_world(p % 7, (p * 3) % 7, float(rng.uniform(-800, -500)), float(rng.uniform(0.8, 1.2)),
+                                        float(rng.uniform(91.25, 100))) for p in range(12)])
+    n, D = 2000, 4
+    variant = rng.integers(0, 12, n).astype(np.uint32)
+    states = rsim.sample_arenas_in(worlds, variant, rsim.SCEN_MIXED, 112)
+    ctl = [rsim.sample_controls(113, d, n) for d in range(D)]
+    dw, dv = device.upload_worlds(worlds), device.upload_words(variant)
+    d = device.upload_states(states)
+    r = device.alloc_words(D * n)
+    device.run(d, dw, device.upload_words(np.concatenate(ctl)), r, D, variant=dv)
+    sample = np.sort(rng.choice(n, 37, replace=False)).astype(np.uint32)
+    g = device.alloc_states(sample.size)
+    rsim._check(rsim.lib().rsim_gather_states(device.handle, d.handle, device.upload_words(sample).handle, g.handle), "gather")
+    device.sync()
+    got, got_r = device.download_states(d), device.read_words(r).reshape(D, n)
+    cpu = states.copy()
+    for k in range(D):
+        assert np.array_equal(oracle.sim8_in(cpu, worlds, variant, ctl[k], threads=4), got_r[k]), "decision %d" % k
+    fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
+    assert fd is None, "arena %d word %d" % fd
+    assert np.array_equal(device.download_states(g), got[sample])

@@ -66,3 +66,26 @@ def test_resting_physics(rsim, oracle):
     assert (np.abs(pos0[:, 2] - 17.0) < 1.0).all()
     ball_z = f32(s[:, 722 + 3 + 9 + 2]) * 50.0
     assert (np.abs(ball_z - 93.0) < 3.0).all()
+
+
+def test_sample_in_worlds_matches_single_world_draws(rsim, oracle):
+    """rsim_sample_arenas_in draws arena i exactly as rsim_sample_arenas does in worlds[variant[i]], and the oracle's
+    per-arena-world batch equals per-arena step_ticks in each arena's world."""
+    import numpy as np
+    import oracle as orc
+
+    worlds = np.stack([rsim.build_world(p, p) for p in (0, 3, 6)])
+    variant = np.array([2, 0, 1, 1, 2, 0, 2, 1, 0, 0, 2, 1], np.uint32)
+    s = rsim.sample_arenas_in(worlds, variant, rsim.SCEN_MIXED, 77, first=5)
+    for i, v in enumerate(variant):
+        assert np.array_equal(s[i], rsim.sample_arenas(worlds[v], rsim.SCEN_MIXED, 77, 1, first=5 + i)[0])
+    ctl = rsim.sample_controls(78, 0, variant.size)
+    a, b = s.copy(), s.copy()
+    r = oracle.sim8_in(a, worlds, variant, ctl, threads=3)
+    for i, v in enumerate(variant):
+        assert r[i] == orc.result_word(*oracle.step_ticks(b[i], worlds[v], ctl[i], 8), b[i])
+    assert np.array_equal(a, b)
+    bad = variant.copy()
+    bad[4] = 3
+    with pytest.raises(rsim.RsimError):
+        rsim.sample_arenas_in(worlds, bad, rsim.SCEN_MIXED, 77)
