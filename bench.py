@@ -226,7 +226,7 @@ def scale_leg(rsim, dev, decisions, block, threads, chunk=50):
     dev.capture_begin()
     for c in range(chunks):
         dev.run(d_s, d_w, d_c[c], d_r[c], chunk, variant=d_v, block=block)
-        rsim._check(rsim.lib().rsim_gather_states(dev.handle, d_s.handle, d_sample.handle, d_g[c].handle), "rsim_gather_states")
+        dev.gather_states(d_s, d_sample, d_g[c])
     graph = dev.capture_end()
     dev.timer_begin()
     graph.launch()

@@ -7,13 +7,15 @@
 namespace rs {
 
 struct SatBatch;
+struct TravBatch;
 // The narrowphase context: the arena, its world, a count of pair algorithms the tick reached that are not implemented, and
-// the tick's precomputed separating-axis tests of the car-mesh jobs (null: every job tests its triangles in place).
+// the tick's precomputed mesh-job BVH walks and car-mesh separating-axis tests (null: every job does its own).
 struct Np {
   ArenaState& s;
   const StaticData& w;
   uint32_t unimplemented;
   SatBatch* sb;
+  TravBatch* tb;
 };
 RS_FN Aff body_trans(const Np& c, uint32_t b) {
   int d = dyn_slot(c.w, b);

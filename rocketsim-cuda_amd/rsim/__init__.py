@@ -388,6 +388,11 @@ class SimDevice:
         _check(lib().rsim_run(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
                               controls.handle, results.handle, decisions, block), "rsim_run")
 
+    def gather_states(self, src, perm, dst):
+        """dst arena i = src arena perm[i] for the dst.n arenas of dst (asynchronous). perm is a DeviceWords; the caller
+        keeps every buffer alive until the work is done (freeing a buffer waits for the stream first)."""
+        _check(lib().rsim_gather_states(self.handle, src.handle, perm.handle, dst.handle), "rsim_gather_states")
+
     def sync(self):
         _check(lib().rsim_sync(self.handle), "rsim_sync")
 

@@ -337,7 +337,8 @@ def test_run_per_arena_worlds_and_sample_gather(rsim, oracle, device):
     device.run(d, dw, device.upload_words(np.concatenate(ctl)), r, D, variant=dv)
     sample = np.sort(rng.choice(n, 37, replace=False)).astype(np.uint32)
     g = device.alloc_states(sample.size)
-    rsim._check(rsim.lib().rsim_gather_states(device.handle, d.handle, device.upload_words(sample).handle, g.handle), "gather")
+    d_sample = device.upload_words(sample)  # held until the gather has run
+    device.gather_states(d, d_sample, g)
     device.sync()
     got, got_r = device.download_states(d), device.read_words(r).reshape(D, n)
     cpu = states.copy()
