@@ -242,11 +242,19 @@ RS_FN int pad_winner(const ArenaState& s, int pad, float mx) {
     if (can_take_boost(s.cars[t.car_idx[q]], mx)) return (int)t.car_idx[q];
   return -1;
 }
+// The steps and their order are the reference's; the pad lists are read together (one bit per pad with a touching car),
+// a list is written back only when it changed, the discovery screen is one mask over the pads' boxes, and the winner is
+// computed for the (at most NUM_CARS) pads a car touches rather than for all 34: the same reads decide the same writes.
 RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_grid.cuh:56-133
+  static_assert(NUM_PADS <= 64, "one bit per pad");
   const float mx = w.mut_f[MF_CAR_MAX_BOOST_AMOUNT];
-  int touching[NUM_CARS], winner[NUM_PADS];
+  int touching[NUM_CARS];
   for (int ci = 0; ci < NUM_CARS; ++ci) touching[ci] = -1;
-  for (int i = 0; i < NUM_PADS; ++i) {  // retention, pad order
+  uint64_t listed_pads = 0;
+  RS_UNROLL
+  for (int i = 0; i < NUM_PADS; ++i) listed_pads |= (uint64_t)(s.pad_touching[i].n != 0u) << i;
+  for (uint64_t m = listed_pads; m != 0; m &= m - 1) {  // retention, pad order (a pad with no car keeps nothing)
+    const int i = __builtin_ctzll(m);
     const PadConst& pad = w.pads[i];
     PadTouching& t = s.pad_touching[i];
     uint32_t kept = 0;
@@ -259,15 +267,21 @@ RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_g
         t.car_idx[kept++] = idx;
       }
     }
-    for (uint32_t q = kept; q < t.n; ++q) t.car_idx[q] = 0;
-    t.n = kept;
+    if (kept != t.n) {
+      for (uint32_t q = kept; q < t.n; ++q) t.car_idx[q] = 0;
+      t.n = kept;
+    }
   }
   for (int ci = 0; ci < NUM_CARS; ++ci) {  // discovery through the cylinder
     const Car& car = s.cars[ci];
     if (car.is_demoed || car.pos.z > w.max_pad_z || touching[ci] >= 0) continue;
-    for (int i = 0; i < NUM_PADS; ++i) {
+    uint64_t in_box = 0;
+    RS_UNROLL
+    for (int i = 0; i < NUM_PADS; ++i)
+      in_box |= (uint64_t)(all3(cmple(w.pads[i].aabb.min, car.pos)) && all3(cmpge(w.pads[i].aabb.max, car.pos))) << i;
+    for (uint64_t m = in_box; m != 0; m &= m - 1) {
+      const int i = __builtin_ctzll(m);
       const PadConst& pad = w.pads[i];
-      if (!(all3(cmple(pad.aabb.min, car.pos)) && all3(cmpge(pad.aabb.max, car.pos)))) continue;
       if (!pad_cyl(pad.pos, pad.cyl_radius, car.pos)) continue;
       PadTouching& t = s.pad_touching[i];
       bool listed = false;
@@ -280,12 +294,13 @@ RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_g
       break;
     }
   }
-  for (int i = 0; i < NUM_PADS; ++i) winner[i] = pad_winner(s, i, mx);
+  int win[NUM_CARS];  // the winner of the pad each car touches, as of now
+  for (int ci = 0; ci < NUM_CARS; ++ci) win[ci] = touching[ci] >= 0 ? pad_winner(s, touching[ci], mx) : -1;
   for (int ci = 0; ci < NUM_CARS; ++ci) {  // pickups, car order; the pad's winner refreshed after a pickup
     Car& car = s.cars[ci];
     if (car.is_demoed || car.boost >= mx) continue;
     int i = touching[ci];
-    if (i < 0 || winner[i] != ci) continue;
+    if (i < 0 || win[ci] != ci) continue;
     const PadConst& pad = w.pads[i];
     int64_t tick = (int64_t)u64(s.tick_count);
     if (s.pad_gave_boost[i].some) {
@@ -296,7 +311,9 @@ RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_g
     s.pad_gave_boost[i].some = 1;
     s.pad_gave_boost[i].v = u64w((uint64_t)tick);
     s.events |= 1u << (EV_PICKUP + ci);
-    winner[i] = pad_winner(s, i, mx);
+    const int nw = pad_winner(s, i, mx);
+    for (int cj = 0; cj < NUM_CARS; ++cj)
+      if (touching[cj] == i) win[cj] = nw;
   }
 }
 RS_FN void phase_p8(ArenaState& s, const StaticData& w) {
