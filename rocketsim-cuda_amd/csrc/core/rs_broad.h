@@ -317,7 +317,16 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
   uint16_t qmin[3], qmax[3];
   sap_quantize(w, box.min, 0, qmin);
   sap_quantize(w, box.max, 1, qmax);
-  for (int ax = 0; ax < 3; ++ax) update_axis(s, w, ax, h, qmin[ax], qmax[ax]);
+  // the handle's six edge positions read together; an axis whose edges keep their positions is left alone, as
+  // update_axis would. Sorting one axis changes only that axis's edges and edge indices, so the other axes' reads stand.
+  const HandleEdges he = s.handle_edges[h];
+  uint16_t pmin[3], pmax[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    pmin[ax] = s.edges[ax][he.min[ax]].pos;
+    pmax[ax] = s.edges[ax][he.max[ax]].pos;
+  }
+  for (int ax = 0; ax < 3; ++ax)
+    if (qmin[ax] != pmin[ax] || qmax[ax] != pmax[ax]) update_axis(s, w, ax, h, qmin[ax], qmax[ax]);
   uint32_t old = s.dyn_proxy[d].cell_idx, ni[3];
   cell_indices(w, box.min, ni);
   uint32_t nidx = cell_index(w, ni);
