@@ -75,6 +75,8 @@ int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** wo
 #define RSIM_MAX_WORLDS 1024
 int rsim_upload_worlds(rsim_device* dev, const void* static_data, uint32_t count, rsim_world** world);
 uint32_t rsim_world_count(const rsim_world* world);
+/* The device address of the world table's first StaticData (for user kernels on include/rsim_device.h). */
+const void* rsim_world_device_ptr(const rsim_world* world);
 int rsim_world_free(rsim_world* world);
 
 /* upload_states / alloc_states (device.rs:73-80): n ArenaStates packed to AoSoA-64 on the host and uploaded; alloc zeroes. */

@@ -13,34 +13,10 @@
 //   k_run                           several decisions per launch, the arena held in the lane from first to last
 //   k_copy_words                    a streaming copy of a state buffer (restores between timed runs)
 #include "device/launch.h"
+#include "device/aosoa.h"
 
 namespace rs {
 namespace dev {
-
-RS_HD inline size_t block_base(uint32_t a) { return (size_t)(a / AOSOA_LANES) * ARENA_WORDS * AOSOA_LANES + (a % AOSOA_LANES); }
-
-__device__ inline void load_all(ArenaState& s, const uint32_t* __restrict__ buf, uint32_t a) {
-  uint32_t* d = reinterpret_cast<uint32_t*>(&s);
-  const uint32_t* src = buf + block_base(a);
-  for (int w = 0; w < ARENA_WORDS; ++w) d[w] = src[(size_t)w * AOSOA_LANES];
-}
-__device__ inline void store_all(uint32_t* __restrict__ buf, uint32_t a, const ArenaState& s) {
-  const uint32_t* d = reinterpret_cast<const uint32_t*>(&s);
-  uint32_t* dst = buf + block_base(a);
-  for (int w = 0; w < ARENA_WORDS; ++w) dst[(size_t)w * AOSOA_LANES] = d[w];
-}
-// Tick-boundary copies: the TICK_WORDS persistent words move; the mid-tick tail is zero between ticks (arena_state.cuh:221).
-__device__ inline void load_tick(ArenaState& s, const uint32_t* __restrict__ buf, uint32_t a) {
-  uint32_t* d = reinterpret_cast<uint32_t*>(&s);
-  const uint32_t* src = buf + block_base(a);
-  for (int w = 0; w < TICK_WORDS; ++w) d[w] = src[(size_t)w * AOSOA_LANES];
-  for (int w = TICK_WORDS; w < ARENA_WORDS; ++w) d[w] = 0;
-}
-__device__ inline void store_tick(uint32_t* __restrict__ buf, uint32_t a, const ArenaState& s) {
-  const uint32_t* d = reinterpret_cast<const uint32_t*>(&s);
-  uint32_t* dst = buf + block_base(a);
-  for (int w = 0; w < TICK_WORDS; ++w) dst[(size_t)w * AOSOA_LANES] = d[w];
-}
 
 __global__ void __launch_bounds__(MAX_BLOCK) k_identity(const uint32_t* in, uint32_t* out, uint32_t n) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;

@@ -218,6 +218,7 @@ int rsim_upload_world(rsim_device* dev, const void* static_data, rsim_world** ou
   return rsim_upload_worlds(dev, static_data, 1, out);
 }
 uint32_t rsim_world_count(const rsim_world* w) { return w ? w->count : 0; }
+const void* rsim_world_device_ptr(const rsim_world* w) { return w ? w->d : nullptr; }
 int rsim_world_free(rsim_world* w) {
   if (!w) return RSIM_OK;
   int rc = use(w->dev);

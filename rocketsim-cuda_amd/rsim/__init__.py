@@ -79,7 +79,7 @@ def lib():
             "rsim_alloc_states": ([_vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_write_states": ([_vp, _vp, _vp], ctypes.c_int), "rsim_copy_states": ([_vp, _vp, _vp], ctypes.c_int),
             "rsim_states_free": ([_vp], ctypes.c_int), "rsim_states_count": ([_vp], u32),
-            "rsim_states_device_ptr": ([_vp], _vp),
+            "rsim_states_device_ptr": ([_vp], _vp), "rsim_world_device_ptr": ([_vp], _vp),
             "rsim_download_words": ([_vp, _vp, _vp], ctypes.c_int), "rsim_download_states": ([_vp, _vp, _vp], ctypes.c_int),
             "rsim_alloc_words": ([_vp, u64, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_upload_words": ([_vp, _vp, u64, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -186,6 +186,11 @@ class DeviceStatic:
 
     def __init__(self, dev, handle):
         self.dev, self.handle = dev, handle
+
+    @property
+    def ptr(self):
+        """Device address of the first StaticData (for user kernels)."""
+        return lib().rsim_world_device_ptr(self.handle)
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit

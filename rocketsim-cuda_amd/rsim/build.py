@@ -3,7 +3,8 @@
     python -m rsim.build            (from rocketsim-cuda_amd/), or rsim.build.build() from Python
 
 Float policy of both compilers: -ffp-contract=off (no fused multiply-add), no fast-math, fp32 denormals kept, HIP's correctly
-rounded float divide and sqrt. All HIP code is compiled without packed-fp32 instructions.
+rounded float divide and sqrt. All HIP code is compiled without packed-fp32 instructions. Everything is compiled with
+-fno-strict-aliasing: the states are copied between buffers and structs through u32 word views.
 """
 import os
 import shutil
@@ -18,9 +19,12 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librsim.so")
 EXAMPLES = {"kickoff": os.path.join(PKG, "examples", "kickoff.cpp")}
+# user kernels on the device API (include/rsim_device.h), each built into its own library beside librsim.so
+DEVICE_EXAMPLES = {"libobserve.so": os.path.join(PKG, "examples", "observe.hip")}
 
 ARCH = "gfx950"
-COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + CSRC, "-I" + INCLUDE]
+# -fno-strict-aliasing: states are copied in and out through u32 word views of the structs
+COMMON = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fno-strict-aliasing", "-I" + CSRC, "-I" + INCLUDE]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,
     "-fno-gpu-flush-denormals-to-zero",
@@ -69,6 +73,10 @@ def build(verbose=False):
         out = os.path.join(HERE, name)
         _run(["g++", "-std=c++17", "-O2", "-I" + INCLUDE, src, "-o", out + ".tmp", "-L" + HERE, "-lrsim",
               "-Wl,-rpath,$ORIGIN"])
+        os.replace(out + ".tmp", out)
+    for name, src in DEVICE_EXAMPLES.items():
+        out = os.path.join(HERE, name)
+        _run([hipcc] + COMMON + HIP_FLAGS + ["-x", "hip", "-shared", src, "-o", out + ".tmp"])
         os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
