@@ -34,7 +34,7 @@ RS_FN V3 orient_pen(V3 a, V3 to_box, V3 face, bool has_face) {
   return a;
 }
 // :525 axis_gap_overlap: the box's radius on axis a against the triangle's projection
-RS_NOINL void gap_overlap(V3 a, const V3 q[3], V3 half, float* gap, float* overlap, float* side) {
+RS_FN void gap_overlap(V3 a, const V3 q[3], V3 half, float* gap, float* overlap, float* side) {
   RS_PRIVATE(q); RS_PRIVATE(gap); RS_PRIVATE(overlap); RS_PRIVATE(side);  // locals of the caller
   float r = (half.x * fabs_(a.x) + half.y * fabs_(a.y)) + half.z * fabs_(a.z);
   float d0 = dot(q[0], a), d1 = dot(q[1], a), d2 = dot(q[2], a);
@@ -48,7 +48,7 @@ RS_NOINL void gap_overlap(V3 a, const V3 q[3], V3 half, float* gap, float* overl
   }
 }
 // :341 consider_axis: true when the axis rejects the pair
-RS_NOINL bool consider_axis(SatSweep* o, V3 a, SatKind kind, const V3 q[3], V3 half, float margin, float maxd) {
+RS_FN bool consider_axis(SatSweep* o, V3 a, SatKind kind, const V3 q[3], V3 half, float margin, float maxd) {
   RS_PRIVATE(o); RS_PRIVATE(q);  // locals of the caller
   float gap, overlap, side;
   gap_overlap(a, q, half, &gap, &overlap, &side);
@@ -135,7 +135,7 @@ RS_FN V3 tri_support(const V3 q[3], V3 dir) {  // :554 first maximum
   }
   return best;
 }
-RS_NOINL int clip_halfspace(const V3* in, int n, V3* out, int axis, float bound, bool keep_above) {  // :607
+RS_FN int clip_halfspace(const V3* in, int n, V3* out, int axis, float bound, bool keep_above) {  // :607
   RS_PRIVATE(in); RS_PRIVATE(out);  // locals of the caller
   if (n == 0) return 0;
   int m = 0;
@@ -195,7 +195,7 @@ RS_FN V3 closest_on_degenerate(V3 p, const V3 q[3]) {  // :669
   return c2;
 }
 // segment-segment closest points (:690), with the near-parallel overlap midpoint rule
-RS_NOINL void segseg_closest(V3 p1, V3 q1, V3 p2, V3 q2, V3* o1, V3* o2) {
+RS_FN void segseg_closest(V3 p1, V3 q1, V3 p2, V3 q2, V3* o1, V3* o2) {
   RS_PRIVATE(o1); RS_PRIVATE(o2);  // locals of the caller
   V3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
   float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
@@ -327,7 +327,7 @@ RS_NOINL bool pen_witness(V3 half, const V3 q[3], SatKind kind, V3 n, V3* res) {
   }
   return false;
 }
-RS_NOINL V3 closest_on_tri(V3 p, const V3 q[3], V3 ab, V3 ac) {  // :848 (Ericson's region test)
+RS_FN V3 closest_on_tri(V3 p, const V3 q[3], V3 ab, V3 ac) {  // :848 (Ericson's region test)
   RS_PRIVATE(q);  // locals of the caller
   V3 ap = p - q[0];
   float d1 = dot(ab, ap), d2 = dot(ac, ap);
@@ -349,7 +349,7 @@ RS_NOINL V3 closest_on_tri(V3 p, const V3 q[3], V3 ab, V3 ac) {  // :848 (Ericso
   float v = vb / den, ww = vc / den;
   return q[0] + ab * v + ac * ww;
 }
-RS_NOINL void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri, V3* o_box) {  // :899
+RS_FN void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri, V3* o_box) {  // :899
   RS_PRIVATE(o_tri); RS_PRIVATE(o_box);  // locals of the caller
   V3 d2 = b1 - b0, r = tp - b0;
   float e = dot(d2, d2), f = dot(d2, r);
