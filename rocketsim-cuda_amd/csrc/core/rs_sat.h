@@ -96,7 +96,7 @@ RS_FN bool box_face(SatSweep* o, float tmin, float tmax, float h, int axis, floa
   return false;
 }
 // :359 sat_sweep; false for V3's None (a rejecting axis)
-RS_NOINL bool sat_sweep(V3 half, const V3 q[3], V3 tmin, V3 tmax, const V3 edges[3], V3 face, bool has_face, V3 centroid, float margin,
+RS_FN bool sat_sweep(V3 half, const V3 q[3], V3 tmin, V3 tmax, const V3 edges[3], V3 face, bool has_face, V3 centroid, float margin,
                         float maxd, SatSweep* o) {
   RS_PRIVATE(q); RS_PRIVATE(edges); RS_PRIVATE(o);  // locals of the caller
   o->has_sep = false;
