@@ -541,7 +541,11 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
     for (int q = 0; q < nc; ++q) found = found || cells[q] == c;
     if (!found) cells[nc++] = c;
   }
-  uint32_t seen = 0;
+  // the screen (first visit, box overlap) over all cells first, then the candidates' ray tests in the same order: a ray
+  // test changes only the hits, which the screen does not read, so this is the sequence of screening and testing entry by
+  // entry, with the screen's loads no longer waiting behind the tests
+  uint32_t seen = 0, cand[32];  // distinct handles: at most one per bit of `seen`
+  int ncand = 0;
   for (int ci = 0; ci < nc; ++ci) {
     uint32_t cell = cells[ci];
     const CellRange& cr = w.cells[cell];
@@ -549,7 +553,7 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       uint32_t h = w.cell_handles[cr.first + e];
       if (seen & (1u << h)) continue;
       seen |= 1u << h;
-      if (overlaps(rb, proxy_box(s, w, h))) ray_candidate(s, w, h, hits);
+      if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
     }
     // the cell's dynamic entries: the list is sorted by cell (an invariant of the format, dump.rs:683-685), so its run
     // is found by bisection and walked in list order, the entries a full scan would visit, in the same order
@@ -564,9 +568,10 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       uint32_t h = s.dyn_cells[e].handle;
       if (seen & (1u << h)) continue;
       seen |= 1u << h;
-      if (overlaps(rb, proxy_box(s, w, h))) ray_candidate(s, w, h, hits);
+      if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
     }
   }
+  for (int k = 0; k < ncand; ++k) ray_candidate(s, w, cand[k], hits);
 }
 
 struct WheelHit {
