@@ -59,6 +59,19 @@ void rsim_cpu_sim8_batch_in(ArenaState* states, const StaticData* worlds, uint32
   }
   for (auto& th : pool) th.join();
 }
+// Byte layout of the mesh records and BVH nodes in StaticData (for tests that relocate a mesh's nodes): offset of
+// meshes, sizeof(MeshInfo), offsets of node_first and node_count in it, offset of nodes, sizeof(WideNode), NUM_MESHES,
+// MAX_NODES.
+void rsim_cpu_static_layout(uint64_t* out) {
+  out[0] = offsetof(StaticData, meshes);
+  out[1] = sizeof(MeshInfo);
+  out[2] = offsetof(MeshInfo, node_first);
+  out[3] = offsetof(MeshInfo, node_count);
+  out[4] = offsetof(StaticData, nodes);
+  out[5] = sizeof(WideNode);
+  out[6] = NUM_MESHES;
+  out[7] = MAX_NODES;
+}
 // Word offset of cars[car].controls in ArenaState (for setting controls before a per-phase run).
 uint64_t rsim_cpu_controls_word(int car) { return (offsetof(ArenaState, cars) + car * sizeof(Car) + offsetof(Car, controls)) / 4; }
 // Brute force: move_cells against the remove-then-push sequence it replaces, on `iters` random sorted lists (lengths up

@@ -47,6 +47,7 @@ class Oracle:
         L.rsim_cpu_sim8_aos.argtypes = [_vp, _vp, _vp, _vp, _u32]
         L.rsim_cpu_sim8_batch.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32]
         L.rsim_cpu_sim8_batch_in.argtypes = [_vp, _vp, _u32, _vp, _vp, _vp, _u32, _u32]
+        L.rsim_cpu_static_layout.argtypes = [_vp]
         L.rsim_cpu_libm_unary.argtypes = [ctypes.c_int, _u64, _u64, _vp]
         L.rsim_cpu_libm_unary_list.argtypes = [ctypes.c_int, _vp, _u64, _vp]
         L.rsim_cpu_libm_binary.argtypes = [ctypes.c_int, _vp, _vp, _u64, _vp]
@@ -98,6 +99,23 @@ class Oracle:
         self.L.rsim_cpu_sim8_batch_in(_p(states), _p(worlds), worlds.reshape(len(worlds), -1).shape[0], _p(variant), _p(controls),
                                       _p(res), n, threads)
         return res[:n]
+
+    def relocate_mesh_nodes(self, world, mesh):
+        """A copy of `world` (u32 words) with mesh `mesh`'s BVH nodes moved to the unused end of the node array and its
+        node_first updated: the same physics, a different node layout. None if the array has no room."""
+        lay = np.zeros(8, np.uint64)
+        self.L.rsim_cpu_static_layout(_p(lay))
+        meshes, msize, nf_off, nc_off, nodes, nsize, nmesh, cap = (int(x) for x in lay)
+        b = np.array(world, np.uint32).view(np.uint8).copy()
+        rec = lambda k, off: meshes + k * msize + off
+        u32 = lambda at: int(b[at:at + 4].view(np.uint32)[0])
+        used = max(u32(rec(k, nf_off)) + u32(rec(k, nc_off)) for k in range(nmesh))
+        first, count = u32(rec(mesh, nf_off)), u32(rec(mesh, nc_off))
+        if count == 0 or used + count > cap:
+            return None
+        b[nodes + used * nsize:nodes + (used + count) * nsize] = b[nodes + first * nsize:nodes + (first + count) * nsize]
+        b[rec(mesh, nf_off):rec(mesh, nf_off) + 4] = np.array([used], np.uint32).view(np.uint8)
+        return b.view(np.uint32)
 
     def check_move_cells(self, world, seed, iters):
         """Mismatches of the one-pass cell move against remove-then-push on random lists (brute force)."""

@@ -347,3 +347,34 @@ def test_run_per_arena_worlds_and_sample_gather(rsim, oracle, device):
     fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
     assert fd is None, "arena %d word %d" % fd
     assert np.array_equal(device.download_states(g), got[sample])
+
+
+def test_pooled_walks_with_mixed_world_layouts(rsim, oracle, device):
+    """Arenas of one wave in worlds with the same physics but different BVH node layouts: each pooled walk must read its
+    own arena's world. Results equal the oracle's and an all-original run's."""
+    note("rsim_run 3 decisions, 4096 contact arenas over two node layouts")
+    a = rsim.build_world(0, 0)
+    b = a
+    for k in (0, 4, 8, 12, 14):
+        b = oracle.relocate_mesh_nodes(b, k)
+    worlds = np.stack([a, b])
+    n, D = 4096, 3
+    rng = np.random.default_rng(193)
+    variant = rng.integers(0, 2, n).astype(np.uint32)
+    states = rsim.sample_arenas(a, rsim.SCEN_CONTACT, 194, n)
+    ctl = [rsim.sample_controls(195, d, n) for d in range(D)]
+    dw, dv = device.upload_worlds(worlds), device.upload_words(variant)
+    d_mixed, d_plain = device.upload_states(states), device.upload_states(states)
+    c = device.upload_words(np.concatenate(ctl))
+    r_mixed, r_plain = device.alloc_words(D * n), device.alloc_words(D * n)
+    device.run(d_mixed, dw, c, r_mixed, D, variant=dv)
+    device.run(d_plain, dw, c, r_plain, D)  # every arena in world 0
+    device.sync()
+    got, got_r = device.download_states(d_mixed), device.read_words(r_mixed).reshape(D, n)
+    assert np.array_equal(got_r, device.read_words(r_plain).reshape(D, n))
+    assert np.array_equal(got[:, :TICK_WORDS], device.download_states(d_plain)[:, :TICK_WORDS])
+    cpu = states.copy()
+    for k in range(D):
+        assert np.array_equal(oracle.sim8_in(cpu, worlds, variant, ctl[k], threads=4), got_r[k]), "decision %d" % k
+    fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
+    assert fd is None, "arena %d word %d" % fd

@@ -89,3 +89,21 @@ def test_sample_in_worlds_matches_single_world_draws(rsim, oracle):
     bad[4] = 3
     with pytest.raises(rsim.RsimError):
         rsim.sample_arenas_in(worlds, bad, rsim.SCEN_MIXED, 77)
+
+
+def test_relocated_mesh_nodes_same_physics(rsim, oracle):
+    """A world whose meshes' BVH nodes sit elsewhere in the node array (node_first changed) steps exactly like the
+    original: the fixture the GPU test of mixed world layouts relies on."""
+    import numpy as np
+
+    a = rsim.build_world(0, 0)
+    b = a
+    for k in (0, 4, 8, 12, 14):
+        b = oracle.relocate_mesh_nodes(b, k)
+        assert b is not None
+    assert (b != a).sum() > 0
+    states = rsim.sample_arenas(a, rsim.SCEN_CONTACT, 191, 64)
+    ctl = rsim.sample_controls(192, 0, 64)
+    sa, sb = states.copy(), states.copy()
+    assert np.array_equal(oracle.sim8(sa, a, ctl, threads=4), oracle.sim8(sb, b, ctl, threads=4))
+    assert np.array_equal(sa, sb)
