@@ -819,64 +819,22 @@ RS_FN bool sat_run_record(const float* r, SatContact* out) {
   V3 q[3] = {v3(r[17], r[18], r[19]), v3(r[20], r[21], r[22]), v3(r[23], r[24], r[25])};
   return box_triangle_sat(box_t, half, r[15], q, r[16], out);
 }
-#if defined(__HIP_DEVICE_COMPILE__)
-// All active lanes of the wave take the collected tests of all of them in turns of one test each: lane ranks and task
-// offsets from ballots, the records staged in the lanes' LDS slots, the results handed back the same way.
-__device__ inline void sat_batch_run(const Np& c, SatBatch& sb) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
-  const uint64_t active = __ballot(1);
-  const uint32_t nact = (uint32_t)__popcll(active), rank = (uint32_t)__popcll(active & below);
-  const uint32_t cnt = sb.n_tasks;
-  uint32_t pre = 0, total = 0;
-  for (uint32_t bit = 0; (1u << bit) <= (uint32_t)SB_TASKS; ++bit) {
-    const uint64_t m = __ballot((cnt >> bit) & 1u);
-    pre += (uint32_t)__popcll(m & below) << bit;
-    total += (uint32_t)__popcll(m) << bit;
-  }
+// All active lanes of the wave take the collected tests of all of them in turns of one test each (wave_pool); on the
+// host, one lane in order.
+RS_FN void sat_batch_run(const Np& c, SatBatch& sb) {
   sb.ok = 0;
-  for (uint32_t base = 0; base < total; base += nact) {  // total and nact are the same in every lane
-    const uint32_t lo = (pre > base ? pre : base) - pre;
-    const uint32_t end = pre + cnt < base + nact ? pre + cnt : base + nact;
-    const uint32_t hi = end > pre ? end - pre : 0u;
-    float r[SB_RECORD];
-    for (uint32_t t = lo; t < hi; ++t) {  // this lane's tasks in the window, into the slots their order gives
-      sat_record(c, sb, t, r);
-      const uint32_t slot = pre + t - base;
-      for (int k = 0; k < SB_RECORD; ++k) *wave_slot(slot, (uint32_t)k) = r[k];
-    }
-    wave_lds_sync();
-    if (base + rank < total) {
-      for (int k = 0; k < SB_RECORD; ++k) r[k] = *wave_slot(rank, (uint32_t)k);
-      SatContact ct = SatContact{};
-      const bool hit = sat_run_record(r, &ct);
-      *wave_slot(rank, 0) = hit ? 1.0f : 0.0f;
-      *wave_slot(rank, 1) = ct.point_world.x;
-      *wave_slot(rank, 2) = ct.point_world.y;
-      *wave_slot(rank, 3) = ct.point_world.z;
-      *wave_slot(rank, 4) = ct.distance;
-    }
-    wave_lds_sync();
-    for (uint32_t t = lo; t < hi; ++t) {
-      const uint32_t slot = pre + t - base;
-      if (*wave_slot(slot, 0) != 0.0f) sb.ok |= 1u << t;
-      sb.res[t] = SatContact{v3(*wave_slot(slot, 1), *wave_slot(slot, 2), *wave_slot(slot, 3)), *wave_slot(slot, 4)};
-    }
-    wave_lds_sync();
-  }
+  wave_pool<SB_RECORD, 5, SB_TASKS>(
+      sb.n_tasks, [&](uint32_t t, float* r) { sat_record(c, sb, t, r); },
+      [](const float* r, float* o) {
+        SatContact ct = SatContact{};
+        o[0] = sat_run_record(r, &ct) ? 1.0f : 0.0f;
+        o[1] = ct.point_world.x, o[2] = ct.point_world.y, o[3] = ct.point_world.z, o[4] = ct.distance;
+      },
+      [&](uint32_t t, const float* o) {
+        if (o[0] != 0.0f) sb.ok |= 1u << t;
+        sb.res[t] = SatContact{v3(o[1], o[2], o[3]), o[4]};
+      });
 }
-#else
-inline void sat_batch_run(const Np& c, SatBatch& sb) {  // one lane: the tests in order
-  sb.ok = 0;
-  float r[SB_RECORD];
-  for (uint32_t t = 0; t < sb.n_tasks; ++t) {
-    sat_record(c, sb, t, r);
-    SatContact ct = SatContact{};
-    if (sat_run_record(r, &ct)) sb.ok |= 1u << t;
-    sb.res[t] = ct;
-  }
-}
-#endif
 
 RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
   const MeshInfo* mesh = mesh_of(c.w, mesh_body);

@@ -70,6 +70,50 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// A wave-pooled stage: this lane's `cnt` tasks (at most MAXT) and every other active lane's are run by all active lanes of
+// the wave in turns of one task per lane. stage(t, rec) writes task t's record (REC words), exec(rec, out) computes its
+// result (OUT words) from the record alone, collect(t, out) hands it back to the owner. Lane ranks and task offsets come
+// from ballots (task order: lane order, then the owner's order); the records and results pass through the LDS slots.
+template <int REC, int OUT, int MAXT, class Stage, class Exec, class Collect>
+__device__ inline void wave_pool(uint32_t cnt, Stage stage, Exec exec, Collect collect) {
+  static_assert(OUT <= REC && REC <= LANE_LDS_WORDS, "a slot holds a record, then its result");
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+  const uint64_t active = __ballot(1);
+  const uint32_t nact = (uint32_t)__popcll(active), rank = (uint32_t)__popcll(active & below);
+  uint32_t pre = 0, total = 0;
+  for (uint32_t bit = 0; (1u << bit) <= (uint32_t)MAXT; ++bit) {
+    const uint64_t m = __ballot((cnt >> bit) & 1u);
+    pre += (uint32_t)__popcll(m & below) << bit;
+    total += (uint32_t)__popcll(m) << bit;
+  }
+  float r[REC];
+  for (uint32_t base = 0; base < total; base += nact) {  // total and nact are the same in every lane
+    const uint32_t lo = (pre > base ? pre : base) - pre;
+    const uint32_t end = pre + cnt < base + nact ? pre + cnt : base + nact;
+    const uint32_t hi = end > pre ? end - pre : 0u;
+    for (uint32_t t = lo; t < hi; ++t) {  // this lane's tasks in the window, into the slots their order gives
+      stage(t, r);
+      const uint32_t slot = pre + t - base;
+      for (int k = 0; k < REC; ++k) *wave_slot(slot, (uint32_t)k) = r[k];
+    }
+    wave_lds_sync();
+    if (base + rank < total) {
+      for (int k = 0; k < REC; ++k) r[k] = *wave_slot(rank, (uint32_t)k);
+      float o[OUT];
+      exec(r, o);
+      for (int k = 0; k < OUT; ++k) *wave_slot(rank, (uint32_t)k) = o[k];
+    }
+    wave_lds_sync();
+    for (uint32_t t = lo; t < hi; ++t) {
+      float o[OUT];
+      const uint32_t slot = pre + t - base;
+      for (int k = 0; k < OUT; ++k) o[k] = *wave_slot(slot, (uint32_t)k);
+      collect(t, o);
+    }
+    wave_lds_sync();
+  }
+}
 #else
 struct DeltaBank {
   V3 v[BANK_BODIES][5];
@@ -90,6 +134,16 @@ struct TravStack {
   }
   uint32_t pop() { return a[--n]; }
 };
+// One lane: the tasks in order.
+template <int REC, int OUT, int MAXT, class Stage, class Exec, class Collect>
+inline void wave_pool(uint32_t cnt, Stage stage, Exec exec, Collect collect) {
+  float r[REC], o[OUT];
+  for (uint32_t t = 0; t < cnt; ++t) {
+    stage(t, r);
+    exec(r, o);
+    collect(t, o);
+  }
+}
 #endif
 
 }  // namespace rs
