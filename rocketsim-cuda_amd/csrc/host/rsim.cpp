@@ -599,3 +599,10 @@ int rsim_step_tick(rsim_device* dev, rsim_states* states, const rsim_world* worl
 }
 
 }  // extern "C"
+
+namespace rs {
+namespace host {
+// The calling thread's rsim_last_error() message, for the host-only parts of the library in other files.
+int set_error(int code, const char* what) { return fail(code, what); }
+}  // namespace host
+}  // namespace rs

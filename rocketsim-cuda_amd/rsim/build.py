@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = [os.path.join(CSRC, "device", "sim_kernels.hip"), os.path.join(CSRC, "device", "cost_model.hip"),
                os.path.join(CSRC, "host", "rsim.cpp")]
-HOST_SOURCES = [os.path.join(CSRC, "host", "world.cpp")]
+HOST_SOURCES = [os.path.join(CSRC, "host", "world.cpp"), os.path.join(CSRC, "host", "v3wire.cpp")]
 
 
 def _hipcc():

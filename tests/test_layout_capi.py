@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "rsim.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("rsim.h", "rsim_v3.h")]
 
 
 def test_layout_constants(rsim):
@@ -45,16 +45,16 @@ def test_pack_round_trip(rsim, n):
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(rsim_\w+)\s*\(", text, re.M)))
 
 
 def test_header_symbols_exported(rsim):
     names = declared_functions()
-    assert len(names) >= 40, names
+    assert len(names) >= 50 and "rsim_v3_parse_arena" in names, names
     L = ctypes.CDLL(rsim.LIB_PATH)
     missing = [n for n in names if not hasattr(L, n)]
-    assert not missing, "declared in include/rsim.h but not exported: %s" % missing
+    assert not missing, "declared in include/ but not exported: %s" % missing
 
 
 def test_errors_without_device(rsim):
@@ -90,3 +90,33 @@ def test_world_builder_same_in_both_builds(rsim, oracle):
     assert np.array_equal(rsim.sample_controls(1, 2, 50), oracle.sample_controls(1, 2, 50))
     # subranges are reproducible on their own
     assert np.array_equal(rsim.sample_arenas(w, 5, 9, 10, first=30), rsim.sample_arenas(w, 5, 9, 40)[30:])
+
+
+def test_named_layout_matches_the_c_layout(rsim, oracle, world):
+    """rsim.layout (the field names of the reference's layout.rs) lays the words out as the C structs do: the landmark
+    offsets the C header asserts, the offsets the oracle's C build reports, and values read through the names."""
+    from rsim import layout as L
+    A, S = L.ARENA, L.STATIC
+    assert A.itemsize == 2784 * 4 and S.itemsize == 317895 * 4 and L.TICK_WORDS == 2567
+    off = lambda dt, name: dt.fields[name][1]
+    assert (off(A, "cars"), off(A, "bodies"), off(A, "manifolds"), off(A, "edges"), off(A, "events")) == (
+        96, 2888, 3552, 9532, 10264)
+    assert (off(S, "tris"), off(S, "nodes"), off(S, "leaves"), off(S, "table_sin")) == (146044, 531004, 981484, 1206044)
+    lay = np.zeros(8, np.uint64)
+    oracle.L.rsim_cpu_static_layout(lay.ctypes.data_as(ctypes.c_void_p))
+    assert [int(x) for x in lay] == [off(S, "meshes"), L.mesh_info.itemsize, off(L.mesh_info, "node_first"),
+                                     off(L.mesh_info, "node_count"), off(S, "nodes"), L.wide_node.itemsize, 16, 3754]
+    for car in (0, 1):
+        assert oracle.controls_word[car] * 4 == off(A, "cars") + car * L.car.itemsize + off(L.car, "controls")
+    sv = L.static_view(world)
+    assert list(sv["dyn_body"]) == [20, 21, 22] and sv["ball_body"] == 20
+    assert sv["table_sin"][4096] == np.float32(1.0) and sv["table_sin"][0] == 0.0  # sin(pi / 2), sin(0)
+    st = rsim.sample_arenas(world, rsim.SCEN_KICKOFF, 1, 4)
+    a = L.arena_view(st)
+    assert a.shape == (4,) and np.all(a["ball"]["pos"]["x"] == 0) and np.all(a["ball"]["pos"]["y"] == 0)
+    assert np.allclose(a["cars"]["boost"], 100.0 / 3.0) and np.all(a["n_active"] == 0)
+    a["cars"][0, 1]["boost"] = 7.5  # a write through the view lands in the words
+    assert st[0].view(np.float32)[(off(A, "cars") + L.car.itemsize + off(L.car, "boost")) // 4] == 7.5
+    leaves = L.arena_leaves()
+    assert len(leaves) == 2784 and leaves[2566] == ("events", "i") and leaves[24] == ("cars.dodge_deadzone", "f")
+    assert leaves[4:7] == [("ball.pos", "f")] * 3 and leaves[-1] == ("records.point.normal_world_on_b", "f")
