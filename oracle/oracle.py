@@ -25,7 +25,7 @@ _u32, _u64 = ctypes.c_uint32, ctypes.c_uint64
 
 def build(ref=None):
     """Compiles liboracle.so (and _ref/libref.so when the reference tree is present, or when ref=True)."""
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so", "liboracle_libm.so"], check=True)
     if ref or (ref is None and os.path.isdir(REF_TREE)):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
@@ -220,6 +220,68 @@ class Reference:
         st = self.L.rscuda_step_ticks(_p(state), _p(world), _p(controls), n, ctypes.byref(sc),
                                       _p(trace) if trace is not None else None)
         return st, sc.value
+
+
+LIBM_FUNCS = ("sin", "cos", "tan", "atan2", "pow", "atan2_64", "hypot64")  # bit f of a libm mask
+PLATFORM_ALL = (1 << len(LIBM_FUNCS)) - 1
+
+
+class MixedOracle:
+    """The oracle's libm attribution build (liboracle_libm.so): the tick with each transcendental call choosing the
+    correctly rounded function or the platform C library by a bit mask (bit f of LIBM_FUNCS set: platform)."""
+
+    def __init__(self, path=os.path.join(HERE, "liboracle_libm.so")):
+        if not os.path.exists(path):
+            raise RuntimeError("liboracle_libm.so is not built: make -C oracle")
+        L = ctypes.CDLL(path)
+        L.rsim_cpu_phase_mixed.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_uint]
+        L.rsim_cpu_step_tick_mixed.argtypes = [_vp, _vp, ctypes.c_uint]
+        L.rsim_cpu_step_ticks_mixed.argtypes = [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32), _vp, ctypes.c_uint]
+        L.rsim_cpu_phase_platform.argtypes = [ctypes.c_int, _vp, _vp]
+        L.rsim_cpu_step_tick_platform.argtypes = [_vp, _vp]
+        L.rsim_cpu_libm_mixed.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_double]
+        L.rsim_cpu_libm_mixed.restype = ctypes.c_double
+        self.L = L
+
+    def phase(self, k, state, world, mask):
+        return self.L.rsim_cpu_phase_mixed(k, _p(state), _p(world), mask)
+
+    def step_tick(self, state, world, mask):
+        return self.L.rsim_cpu_step_tick_mixed(_p(state), _p(world), mask)
+
+    def step_ticks(self, state, world, controls, n, mask, trace=None):
+        sc = _u32(0)
+        st = self.L.rsim_cpu_step_ticks_mixed(_p(state), _p(world), _p(controls), n, ctypes.byref(sc),
+                                              _p(trace) if trace is not None else None, mask)
+        return st, sc.value
+
+    def libm(self, fn, mask, x, y=0.0):
+        return self.L.rsim_cpu_libm_mixed(fn, mask, x, y)
+
+
+class RefMixed:
+    """The reference's own mixed and platform builds (phase_mixed.cpp, phase_platform.cpp, compiled in place)."""
+
+    def __init__(self, mixed=os.path.join(HERE, "_ref", "libref_mixed.so"),
+                 platform=os.path.join(HERE, "_ref", "libref_platform.so")):
+        for p in (mixed, platform):
+            if not os.path.exists(p):
+                raise RuntimeError("%s is not built (make -C oracle ref, needs the reference tree)" % p)
+        M, P = ctypes.CDLL(mixed), ctypes.CDLL(platform)
+        M.rscuda_phase_mixed.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_uint]
+        M.rscuda_step_tick_mixed.argtypes = [_vp, _vp, ctypes.c_uint]
+        P.rscuda_phase_platform.argtypes = [ctypes.c_int, _vp, _vp]
+        P.rscuda_step_tick_platform.argtypes = [_vp, _vp]
+        self.M, self.P = M, P
+
+    def phase(self, k, state, world, mask):
+        return self.M.rscuda_phase_mixed(k, _p(state), _p(world), mask)
+
+    def step_tick(self, state, world, mask):
+        return self.M.rscuda_step_tick_mixed(_p(state), _p(world), mask)
+
+    def step_tick_platform(self, state, world):
+        return self.P.rscuda_step_tick_platform(_p(state), _p(world))
 
 
 def result_word(status, scored, state):

@@ -231,7 +231,7 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
 
 // ---- auto-flip roll sign (car_pre_tick.cuh:23-72) ------------------------------------------------------------------------------
 RS_FN int32_t axis_angle_to_int(float a0, float a1) {  // car/base.rs axis_angle_to_int
-  double ang = crm::atan2_64((double)a0, (double)a1);
+  double ang = datan2((double)a0, (double)a1);
   float u = ((float)ang * 32768.0f) / k::PI_F;
   return f2i_rs(sse_round_even(u * 2.0f + 0.5f)) >> 1;
 }
@@ -244,7 +244,7 @@ RS_FN V3 rebuild_row1(const StaticData& w, int32_t pitch, int32_t yaw) {  // car
 }
 RS_FN float autoflip_sign(const StaticData& w, const M3& rot) {
   int32_t yaw = axis_angle_to_int(rot.c0.y, rot.c0.x);
-  float len = (float)crm::hypot64((double)rot.c0.x, (double)rot.c0.y);
+  float len = (float)dhypot((double)rot.c0.x, (double)rot.c0.y);
   int32_t pitch = axis_angle_to_int(rot.c0.z, len);
   V3 r1 = rebuild_row1(w, pitch, yaw);
   int16_t roll = (int16_t)axis_angle_to_int(dot(rot.c2, r1), dot(rot.c1, r1));

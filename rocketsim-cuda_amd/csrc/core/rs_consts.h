@@ -98,12 +98,33 @@ constexpr uint32_t FILTER_DEFAULT = 1u, FILTER_STATIC = 1u << 1, FILTER_DROPSHOT
 
 }  // namespace k
 
-// The transcendental calls of the tick (libm.cuh:40-69), correctly rounded.
-RS_FN float fsin(float x) { return crm::sinf_cr(x); }
-RS_FN float fcos(float x) { return crm::cosf_cr(x); }
-RS_FN float ftan(float x) { return crm::tanf_cr(x); }
-RS_FN float fatan2(float y, float x) { return crm::atan2f_cr(y, x); }
-RS_FN float fpow(float x, float y) { return crm::powf_cr(x, y); }
+// The transcendental calls of the tick (libm.cuh:40-69), correctly rounded, so that every build agrees bit for bit. Two
+// host-only builds of the oracle choose otherwise, to attribute differences from V3 to the libm (libm.cuh:7-12):
+// RS_LIBM_PLATFORM makes all seven the platform C library as V3's Rust reaches it (sinf, cosf, tanf, atan2f, powf, atan2,
+// hypot); RS_LIBM_MIXED chooses per call from the thread-local mask libm_platform_mask (bit 0 sin, 1 cos, 2 tan, 3 atan2,
+// 4 pow, 5 atan2_64, 6 hypot64; a set bit is the platform library). The product defines neither.
+#if defined(RS_LIBM_MIXED) && defined(RS_LIBM_PLATFORM)
+#error "define at most one of RS_LIBM_MIXED and RS_LIBM_PLATFORM"
+#endif
+#if (defined(RS_LIBM_MIXED) || defined(RS_LIBM_PLATFORM)) && defined(__HIP_DEVICE_COMPILE__)
+#error "the platform-libm builds are host-only"
+#endif
+#if defined(RS_LIBM_MIXED)
+inline thread_local uint32_t libm_platform_mask = 0;
+#define RS_LIBM_PICK(bit, platform, cr) ((libm_platform_mask & (bit)) ? (platform) : (cr))
+#elif defined(RS_LIBM_PLATFORM)
+#define RS_LIBM_PICK(bit, platform, cr) (platform)
+#else
+#define RS_LIBM_PICK(bit, platform, cr) (cr)
+#endif
+RS_FN float fsin(float x) { return RS_LIBM_PICK(1u, ::sinf(x), crm::sinf_cr(x)); }
+RS_FN float fcos(float x) { return RS_LIBM_PICK(2u, ::cosf(x), crm::cosf_cr(x)); }
+RS_FN float ftan(float x) { return RS_LIBM_PICK(4u, ::tanf(x), crm::tanf_cr(x)); }
+RS_FN float fatan2(float y, float x) { return RS_LIBM_PICK(8u, ::atan2f(y, x), crm::atan2f_cr(y, x)); }
+RS_FN float fpow(float x, float y) { return RS_LIBM_PICK(16u, ::powf(x, y), crm::powf_cr(x, y)); }
+// The f64 pair of the auto-flip path (car_pre_tick.cuh:23-72): a double-double atan2 and the correctly rounded hypot.
+RS_FN double datan2(double y, double x) { return RS_LIBM_PICK(32u, ::atan2(y, x), crm::atan2_64(y, x)); }
+RS_FN double dhypot(double x, double y) { return RS_LIBM_PICK(64u, ::hypot(x, y), crm::hypot64(x, y)); }
 // f32::powi is LLVM's square-and-multiply (libm.cuh:54-65).
 RS_FN float fpowi(float a, int b) {
   unsigned p = b < 0 ? 0u - (unsigned)b : (unsigned)b;
