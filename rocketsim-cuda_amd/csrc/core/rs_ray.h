@@ -4,11 +4,11 @@
 #pragma once
 #include "rs_body.h"
 #include "rs_stack.h"
+#include "rs_trav.h"
 
 namespace rs {
 
 constexpr uint32_t NO_BODY = 0xFFFFFFFFu;
-constexpr uint32_t BVH_LEAF = 1u << 31;
 
 RS_FN int dyn_slot(const StaticData& w, uint32_t body) {  // state slot of a body, -1 for a static one
   for (int d = 0; d < NUM_DYN; ++d)
@@ -403,16 +403,6 @@ RS_FN void plane_rays(const Plane& pl, BodyRays& br, const RayPacket& rp) {  // 
     if (hm & (1u << l)) body_report(br, n, t[l] / dist[l], l);
 }
 
-// BVH4 node test (bvh.cuh:16-25): lane l set when child l's box overlaps `box` (touching counts)
-RS_FN uint32_t node_mask(const WideNode& n, const Aabb& b) {
-  uint32_t m = 0;
-  for (int l = 0; l < 4; ++l) {
-    bool o = n.min_x[l] <= b.max.x && n.max_x[l] >= b.min.x && n.min_y[l] <= b.max.y && n.max_y[l] >= b.min.y &&
-             n.min_z[l] <= b.max.z && n.max_z[l] >= b.min.z;
-    m |= (o ? 1u : 0u) << l;
-  }
-  return m & n.valid_mask;
-}
 // mesh rays: depth-first BVH4 walk, overlapping children pushed in reverse lane order (quad_ray_shapes.cuh:377-406)
 RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, RayPacket& rp) {
   if (!overlaps(rp.box, mesh.root_aabb)) return;
@@ -439,6 +429,23 @@ RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, Ra
       if (mask & (1u << lane)) st.push(node.children[lane]);
   }
 }
+// mesh rays over a leaf list walked beforehand: the leaves mesh_rays' own walk reaches (its node test reads only the
+// packet's box), in the same depth-first order, each tested against the rays as they stand when it is reached
+RS_FN void mesh_rays_leaves(const StaticData& w, const MeshInfo& mesh, BodyRays& br, RayPacket& rp, const uint32_t* leaves,
+                            uint32_t count) {
+  if (count == 0) return;
+  float org[3][4], inv[3][4];
+  packet_pos_dir(rp, org, inv);
+  for (uint32_t k = 0; k < count; ++k) {
+    const Leaf& leaf = w.leaves[mesh.leaf_first + leaves[k]];
+    uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
+    if (mask != 0) {
+      const Tri& tri = w.tris[mesh.tri_first + leaf.tri];
+      for (int i = 0; i < 4; ++i)
+        if (mask & (1u << i)) ray_triangle(br, tri, &rp.lambda_max[i], i);
+    }
+  }
+}
 
 // The first mesh (plane) record at or after `from` that belongs to `body`; NUM_MESHES (NUM_PLANES) when none does.
 RS_FN int next_mesh_of(const StaticData& w, uint32_t body, int from) {
@@ -457,9 +464,9 @@ RS_FN int next_plane_of(const StaticData& w, uint32_t body, int from) {
 }
 
 // collision_world.rs quad_ray_test (raycaster.cuh:20-70): the rays into the body's frame, then its shape
-RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, ClosestHits& h) {
+RS_FN void body_ray_setup(const ArenaState& s, const StaticData& w, uint32_t body, ClosestHits& h, BodyRays& br,
+                          RayPacket& rp) {
   BodyView co = body_view(s, w, body);
-  const BodyConst& bc = w.bodies[body];
   const Aff& tr = co.world;
   V3 from[4], to[4];
   const M3& m = tr.m;
@@ -477,7 +484,6 @@ RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, Cl
       to[i] = xform_point(inv, h.to[i]);
     }
   }
-  BodyRays br;
   for (int i = 0; i < 4; ++i) {
     br.from[i] = from[i];
     br.to[i] = to[i];
@@ -486,9 +492,19 @@ RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, Cl
   br.rot = tr.m;
   br.body = body;
   br.hits = &h;
-  RayPacket rp;
   packet_init(rp, from, to);
   for (int i = 0; i < 4; ++i) rp.lambda_max[i] = br.frac[i];
+}
+RS_FN bool rays_walk_mesh(uint32_t shape_kind) {
+  return shape_kind != k::SHAPE_COMPOUND && shape_kind != k::SHAPE_SPHERE && shape_kind != k::SHAPE_PLANE;
+}
+// `tb` (may be null): mesh walks done beforehand, taken in order through *next_job (grid_ray_test collects them)
+RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, ClosestHits& h, const TravBatch* tb,
+                     uint32_t* next_job) {
+  const BodyConst& bc = w.bodies[body];
+  BodyRays br;
+  RayPacket rp;
+  body_ray_setup(s, w, body, h, br, rp);
   if (bc.shape_kind == k::SHAPE_COMPOUND) {
     compound_rays(bc, br, rp);
   } else if (bc.shape_kind == k::SHAPE_SPHERE) {
@@ -496,9 +512,14 @@ RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, Cl
   } else if (bc.shape_kind == k::SHAPE_PLANE) {
     for (int p = next_plane_of(w, body, 0); p < NUM_PLANES; p = next_plane_of(w, body, p + 1)) plane_rays(w.planes[p], br, rp);
   } else {
-    // every mesh record of the body, in record order; one call site, so lanes walking different meshes walk together
-    for (int mi = next_mesh_of(w, body, 0); mi < NUM_MESHES; mi = next_mesh_of(w, body, mi + 1))
-      mesh_rays(w, w.meshes[mi], br, rp);
+    // every mesh record of the body, in record order, from its walked leaf list when the batch holds one
+    for (int mi = next_mesh_of(w, body, 0); mi < NUM_MESHES; mi = next_mesh_of(w, body, mi + 1)) {
+      const TravJob* jb = nullptr;
+      if (tb && *next_job < tb->n_jobs && tb->job[*next_job].body == body && tb->job[*next_job].mesh_body == (uint32_t)mi)
+        jb = &tb->job[(*next_job)++];
+      if (jb && !jb->in_place) mesh_rays_leaves(w, w.meshes[mi], br, rp, tb->leaf_tri + jb->first, jb->count);
+      else mesh_rays(w, w.meshes[mi], br, rp);
+    }
   }
 }
 
@@ -518,10 +539,13 @@ RS_FN Aabb proxy_box(const ArenaState& s, const StaticData& w, uint32_t h) {
     if (w.dyn_handle[d] == h) return s.dyn_proxy[d].aabb;
   return w.proxies[h].aabb;
 }
-RS_FN void ray_candidate(const ArenaState& s, const StaticData& w, uint32_t h, ClosestHits& hits) {  // :89-95
+RS_FN bool ray_candidate_passes(const ProxyConst& p, const ClosestHits& hits) {  // :89-95
+  return hits.ignore_body != p.client_body && (p.filter_group & hits.mask) != 0 && (hits.group & p.filter_mask) != 0;
+}
+RS_FN void ray_candidate(const ArenaState& s, const StaticData& w, uint32_t h, ClosestHits& hits, const TravBatch* tb,
+                         uint32_t* next_job) {
   const ProxyConst& p = w.proxies[h];
-  if (hits.ignore_body == p.client_body) return;
-  if ((p.filter_group & hits.mask) != 0 && (hits.group & p.filter_mask) != 0) body_rays(s, w, p.client_body, hits);
+  if (ray_candidate_passes(p, hits)) body_rays(s, w, p.client_body, hits, tb, next_job);
 }
 // grid_broadphase.rs ray_test (raycaster.cuh:100-148): the cells of the four ray sources in ray order; in each cell its
 // static handles, then its dynamic ones; every handle at most once.
@@ -571,7 +595,29 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
     }
   }
-  for (int k = 0; k < ncand; ++k) ray_candidate(s, w, cand[k], hits);
+  // the candidates' mesh walks, in the order the tests below reach them, walked together first (on the GPU pooled across
+  // the wave's lanes); a walk's node test reads only the packet's box in the mesh frame, which the hits do not change
+  TravBatch tb;
+  tb.n_jobs = tb.n_leaves = tb.next = 0;
+  tb.leaves = 1;
+  for (int k = 0; k < ncand && tb.n_jobs < (uint32_t)TJ_MAX; ++k) {
+    const ProxyConst& p = w.proxies[cand[k]];
+    const uint32_t body = p.client_body;
+    if (!ray_candidate_passes(p, hits) || !rays_walk_mesh(w.bodies[body].shape_kind)) continue;
+    BodyRays br;
+    RayPacket rp;
+    body_ray_setup(s, w, body, hits, br, rp);
+    for (int mi = next_mesh_of(w, body, 0); mi < NUM_MESHES && tb.n_jobs < (uint32_t)TJ_MAX; mi = next_mesh_of(w, body, mi + 1)) {
+      TravJob& jb = tb.job[tb.n_jobs++];
+      jb.kind = TJ_RAYS;
+      jb.body = body;
+      jb.mesh_body = (uint32_t)mi;
+      jb.box = rp.box;
+    }
+  }
+  trav_run(w, tb);
+  uint32_t next_job = 0;
+  for (int k = 0; k < ncand; ++k) ray_candidate(s, w, cand[k], hits, &tb, &next_job);
 }
 
 struct WheelHit {
