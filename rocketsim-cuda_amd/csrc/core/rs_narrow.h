@@ -437,9 +437,7 @@ RS_FN bool sphere_triangle(const Tri& t, V3 ctr, float radius, float rwt, float 
   *point = cp;
   return true;
 }
-// sphere_concave_collision_alg.rs process_collision_into: every BVH leaf near the ball, in traversal order, adds its contact to
-// the pair's retained manifold; then the manifold is refreshed.
-// ---- the tick's mesh-job BVH walks, ahead of the pair loop ------------------------------------------------------------------
+// ---- the tick's mesh-job BVH walks, ahead of the pair loop (rs_trav.h) -----------------------------------------------------
 // The job record of (kind, body, mesh body), searched from the last one taken (jobs are taken in collection order).
 RS_FN const TravJob* trav_find(const Np& c, uint32_t kind, uint32_t body, uint32_t mesh_body) {
   if (!c.tb) return nullptr;
@@ -453,6 +451,8 @@ RS_FN const TravJob* trav_find(const Np& c, uint32_t kind, uint32_t body, uint32
   return nullptr;
 }
 
+// sphere_concave_collision_alg.rs process_collision_into: every BVH leaf near the ball, in traversal order, adds its contact to
+// the pair's retained manifold; then the manifold is refreshed.
 RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
   Aff bt = body_trans(c, ball), mt = body_trans(c, mesh_body);
   const MeshInfo* mesh = mesh_of(c.w, mesh_body);

@@ -50,9 +50,9 @@ RS_FN bool cursor_next(BvhCursor& cur, const StaticData& w, const MeshInfo& mesh
   return true;
 }
 
-// Each mesh job (a car box or the ball against an arena mesh) walks the mesh's BVH4 depth first and visits the overlapping
-// leaves in that order. P3b collects the tick's jobs first and walks them all before the pair loop; the jobs then take
-// their leaves from the list instead of walking. On the GPU the walks of all active lanes of a wave are pooled: a
+// Each mesh job (a car box or the ball against an arena mesh in P3b, a wheel-ray packet's box against a candidate mesh in
+// P1) walks the mesh's BVH4 depth first and visits the overlapping leaves in that order. The caller collects its jobs
+// first and walks them all as one batch; the jobs then take their leaves from the list instead of walking. On the GPU the walks of all active lanes of a wave are pooled: a
 // level-by-level walk over a frontier shared through LDS, every lane testing one node of any job at a time, each item
 // carrying its path (the child slot at each level) as a key; the leaves of a job, ranked by key, are its depth-first
 // order. Anything that does not fit (frontier, leaf list, depth, a partial wave) falls back to each lane's own walks.
