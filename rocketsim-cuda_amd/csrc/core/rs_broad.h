@@ -107,73 +107,89 @@ RS_FN bool overlap_2d(const ArenaState& s, uint32_t ha, uint32_t hb, int axis) {
 }
 // The four incremental sorts (axis_sweep.cuh:49-127): the moved edge bubbles past its neighbours, each passed neighbour's
 // edge index is adjusted, and a min edge passing a max edge (or the reverse) adds or removes the pair per direction.
+// The moving edge is held in registers and written once at its final slot, and its own edge index is adjusted once by
+// the number of slots passed: nothing inside the loop reads either (the overlap tests read the other two axes, the pair
+// and manifold updates no edges), so this is the reference's sequence of swaps, with a neighbour's load per step as the
+// only memory on the loop's critical path.
 RS_FN void sort_min_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
   Edge* e = s.edges[ax];
-  uint32_t mv = e[i].handle;
-  while (i > 0 && e[i].pos < e[i - 1].pos) {
-    Edge prev = e[i - 1];
+  const Edge cur = e[i];
+  const uint32_t mv = cur.handle, i0 = i;
+  while (i > 0) {
+    const Edge prev = e[i - 1];
+    if (!(cur.pos < prev.pos)) break;
     if (edge_max(prev)) {
       if (overlap_2d(s, mv, prev.handle, ax)) add_pair(s, w, mv, prev.handle);
       s.handle_edges[prev.handle].max[ax] += 1;
     } else {
       s.handle_edges[prev.handle].min[ax] += 1;
     }
-    s.handle_edges[mv].min[ax] -= 1;
-    e[i - 1] = e[i];
     e[i] = prev;
     --i;
   }
+  if (i == i0) return;
+  e[i] = cur;
+  s.handle_edges[mv].min[ax] = (uint16_t)(s.handle_edges[mv].min[ax] - (uint16_t)(i0 - i));
 }
 RS_FN void sort_max_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
   Edge* e = s.edges[ax];
-  uint32_t mv = e[i].handle;
-  while (i + 1 < (uint32_t)NUM_EDGES && e[i + 1].handle != 0 && e[i].pos >= e[i + 1].pos) {
-    Edge next = e[i + 1];
+  const Edge cur = e[i];
+  const uint32_t mv = cur.handle, i0 = i;
+  while (i + 1 < (uint32_t)NUM_EDGES) {
+    const Edge next = e[i + 1];
+    if (!(next.handle != 0 && cur.pos >= next.pos)) break;
     if (!edge_max(next)) {
       if (overlap_2d(s, mv, next.handle, ax)) add_pair(s, w, mv, next.handle);
       s.handle_edges[next.handle].min[ax] -= 1;
     } else {
       s.handle_edges[next.handle].max[ax] -= 1;
     }
-    s.handle_edges[mv].max[ax] += 1;
-    e[i + 1] = e[i];
     e[i] = next;
     ++i;
   }
+  if (i == i0) return;
+  e[i] = cur;
+  s.handle_edges[mv].max[ax] = (uint16_t)(s.handle_edges[mv].max[ax] + (uint16_t)(i - i0));
 }
 RS_FN void sort_min_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
   Edge* e = s.edges[ax];
-  uint32_t mv = e[i].handle;
-  while (i + 1 < (uint32_t)NUM_EDGES && e[i + 1].handle != 0 && e[i].pos >= e[i + 1].pos) {
-    Edge next = e[i + 1];
+  const Edge cur = e[i];
+  const uint32_t mv = cur.handle, i0 = i;
+  while (i + 1 < (uint32_t)NUM_EDGES) {
+    const Edge next = e[i + 1];
+    if (!(next.handle != 0 && cur.pos >= next.pos)) break;
     if (edge_max(next)) {
       if (overlap_2d(s, mv, next.handle, ax)) remove_pair(s, w, mv, next.handle);
       s.handle_edges[next.handle].max[ax] -= 1;
     } else {
       s.handle_edges[next.handle].min[ax] -= 1;
     }
-    s.handle_edges[mv].min[ax] += 1;
-    e[i + 1] = e[i];
     e[i] = next;
     ++i;
   }
+  if (i == i0) return;
+  e[i] = cur;
+  s.handle_edges[mv].min[ax] = (uint16_t)(s.handle_edges[mv].min[ax] + (uint16_t)(i - i0));
 }
 RS_FN void sort_max_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
   Edge* e = s.edges[ax];
-  uint32_t mv = e[i].handle;
-  while (i > 0 && e[i].pos < e[i - 1].pos) {
-    Edge prev = e[i - 1];
+  const Edge cur = e[i];
+  const uint32_t mv = cur.handle, i0 = i;
+  while (i > 0) {
+    const Edge prev = e[i - 1];
+    if (!(cur.pos < prev.pos)) break;
     if (!edge_max(prev)) {
       if (overlap_2d(s, mv, prev.handle, ax)) remove_pair(s, w, mv, prev.handle);
       s.handle_edges[prev.handle].min[ax] += 1;
     } else {
       s.handle_edges[prev.handle].max[ax] += 1;
     }
-    s.handle_edges[mv].max[ax] -= 1;
-    e[i - 1] = e[i];
     e[i] = prev;
     --i;
   }
+  if (i == i0) return;
+  e[i] = cur;
+  s.handle_edges[mv].max[ax] = (uint16_t)(s.handle_edges[mv].max[ax] - (uint16_t)(i0 - i));
 }
 RS_FN void sap_quantize(const StaticData& w, V3 p, uint16_t is_max, uint16_t out[3]) {  // axis_sweep.cuh:16-24
   V3 sc = (p - w.sap_world_min) * w.sap_quantize;
