@@ -466,12 +466,25 @@ RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
   Aabb box = sphere_box(radius, ctr);
   const TravJob* jb = trav_find(c, TJ_BALL, ball, mesh_body);
   if (jb && !jb->in_place) {  // the walk ran ahead: its leaves in depth-first order
-    for (uint32_t k = 0; k < jb->count; ++k) {
-      const uint32_t tri = c.tb->leaf_tri[jb->first + k];
+    // the next leaf's triangle is loaded before this one's test and contact (loads only: the same sequence of tests)
+    const uint32_t* leaves = c.tb->leaf_tri + jb->first;
+    const uint32_t count = jb->count;
+    uint32_t tri_next = 0u;
+    Tri t_next = {};
+    if (count) {
+      tri_next = leaves[0];
+      t_next = c.w.tris[mesh->tri_first + tri_next];
+    }
+    for (uint32_t k = 0; k < count; ++k) {
+      const uint32_t tri = tri_next;
+      const Tri t = t_next;
+      if (k + 1 < count) {
+        tri_next = leaves[k + 1];
+        t_next = c.w.tris[mesh->tri_first + tri_next];
+      }
       V3 n, p;
       float depth;
-      if (sphere_triangle(c.w.tris[mesh->tri_first + tri], ctr, radius, rwt, rwt2, &n, &p, &depth))
-        add_contact(c, m, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
+      if (sphere_triangle(t, ctr, radius, rwt, rwt2, &n, &p, &depth)) add_contact(c, m, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
     }
   } else {
     BvhCursor cur;

@@ -436,8 +436,11 @@ RS_FN void mesh_rays_leaves(const StaticData& w, const MeshInfo& mesh, BodyRays&
   if (count == 0) return;
   float org[3][4], inv[3][4];
   packet_pos_dir(rp, org, inv);
+  // the next leaf is loaded before this one's tests (loads only: the same sequence of tests)
+  Leaf next = w.leaves[mesh.leaf_first + leaves[0]];
   for (uint32_t k = 0; k < count; ++k) {
-    const Leaf& leaf = w.leaves[mesh.leaf_first + leaves[k]];
+    const Leaf leaf = next;
+    if (k + 1 < count) next = w.leaves[mesh.leaf_first + leaves[k + 1]];
     uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
     if (mask != 0) {
       const Tri& tri = w.tris[mesh.tri_first + leaf.tri];
