@@ -73,16 +73,21 @@ def stepped_ticks(results):
     return sum(int(np.where((r & 0xFF) > 0, r & 0xFF, 8).sum()) for r in results)
 
 
+LABEL_ARENAS, LABEL_DECISIONS = 65536, 21  # per scenario mix: 3 x 65,536 x 21 = 4,128,768 labelled arena-decisions
+
+
 def train_cost_model(rsim, dev, world_host, dw):
-    """The cost model trained on labels the simulator emits: 8,192 arenas of each of three scenario mixes (seeds apart
-    from every timed batch) stepped 6 decisions with one arena per wave and each lane's clock recorded."""
+    """The cost model trained on labels the simulator emits: 65,536 arenas of each of three scenario mixes (seeds apart
+    from every timed batch) stepped 21 decisions with one arena per wave and each lane's clock recorded, 4.13 M
+    arena-decisions in all."""
     import rsim.cost as cm
 
     t0 = time.perf_counter()
     F, T, C = [], [], []
     for scen, seed in ((rsim.SCEN_CONTACT, 77), (rsim.SCEN_OPEN_CONTACT, 79), (rsim.SCEN_MIXED, 81)):
-        f, t, c = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, scen, seed, 8192),
-                                 [rsim.sample_controls(seed + 1, d, 8192) for d in range(6)], decisions=6)
+        f, t, c = cm.label_costs(dev, dw, rsim.sample_arenas(world_host, scen, seed, LABEL_ARENAS),
+                                 [rsim.sample_controls(seed + 1, d, LABEL_ARENAS) for d in range(LABEL_DECISIONS)],
+                                 decisions=LABEL_DECISIONS)
         F.append(f), T.append(t), C.append(c)
     feats, targets, contacts = np.concatenate(F), np.concatenate(T), np.concatenate(C)
     label_s = time.perf_counter() - t0

@@ -100,6 +100,10 @@ int rsim_write_words(rsim_device* dev, rsim_words* dst, const uint32_t* host, ui
 int rsim_read_words(rsim_device* dev, const rsim_words* src, uint32_t* host, uint64_t words);
 int rsim_upload_controls(rsim_device* dev, const void* controls, uint32_t n, rsim_words** out);
 int rsim_words_free(rsim_words* buf);
+/* Words offset..offset+words-1 of `parent` as a buffer of their own, for any call that takes a words buffer (no copy;
+ * `parent` must outlive it; freeing the view leaves the memory alone). No counterpart in the reference: it lets a
+ * caller step through one device-resident array in slices, e.g. the cost model's training batches. */
+int rsim_words_view(rsim_words* parent, uint64_t offset, uint64_t words, rsim_words** out);
 uint64_t rsim_words_count(const rsim_words* buf);
 void* rsim_words_device_ptr(rsim_words* buf);
 

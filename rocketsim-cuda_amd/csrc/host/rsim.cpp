@@ -40,6 +40,7 @@ struct rsim_words {
   rsim_device* dev;
   uint32_t* d;
   uint64_t n;
+  bool view = false;  // a range of another buffer: freeing it leaves the memory alone
 };
 struct rsim_graph {
   rsim_device* dev;
@@ -370,11 +371,23 @@ int rsim_upload_controls(rsim_device* dev, const void* controls, uint32_t n, rsi
 }
 int rsim_words_free(rsim_words* b) {
   if (!b) return RSIM_OK;
+  if (b->view) {
+    delete b;
+    return RSIM_OK;
+  }
   int rc = use(b->dev);
   if (rc) return rc;
   (void)hipStreamSynchronize(b->dev->stream);
   (void)hipFree(b->d);
   delete b;
+  return RSIM_OK;
+}
+int rsim_words_view(rsim_words* parent, uint64_t offset, uint64_t words, rsim_words** out) {
+  if (!parent || !out) return fail(RSIM_ERR_INVALID, "rsim_words_view: null argument");
+  if (offset > parent->n || words > parent->n - offset) return fail(RSIM_ERR_SHORT, "rsim_words_view: range past the buffer");
+  rsim_words* v = new (std::nothrow) rsim_words{parent->dev, parent->d + offset, words, true};
+  if (!v) return fail(RSIM_ERR_INVALID, "out of host memory");
+  *out = v;
   return RSIM_OK;
 }
 uint64_t rsim_words_count(const rsim_words* b) { return b ? b->n : 0; }

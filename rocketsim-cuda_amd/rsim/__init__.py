@@ -86,6 +86,7 @@ def lib():
             "rsim_write_words": ([_vp, _vp, _vp, u64], ctypes.c_int), "rsim_read_words": ([_vp, _vp, _vp, u64], ctypes.c_int),
             "rsim_upload_controls": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_words_free": ([_vp], ctypes.c_int), "rsim_words_count": ([_vp], u64), "rsim_words_device_ptr": ([_vp], _vp),
+            "rsim_words_view": ([_vp, u64, u64, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_sim8": ([_vp, _vp, _vp, _vp, _vp, u32, _vp], ctypes.c_int),
             "rsim_identity": ([_vp, _vp, _vp], ctypes.c_int), "rsim_identity_tick": ([_vp, _vp, _vp], ctypes.c_int),
             "rsim_phase": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
@@ -256,6 +257,14 @@ class DeviceWords:
     @property
     def ptr(self):
         return lib().rsim_words_device_ptr(self.handle)
+
+    def view(self, offset, n):
+        """Words offset..offset+n-1 as a DeviceWords of their own (no copy; keeps this buffer alive)."""
+        h = _vp()
+        _check(lib().rsim_words_view(self.handle, offset, n, ctypes.byref(h)), "rsim_words_view")
+        v = DeviceWords(self.dev, h.value, n)
+        v.parent = self
+        return v
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit

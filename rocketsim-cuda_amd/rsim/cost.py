@@ -69,28 +69,36 @@ class CostModel:
         return scratch, loss
 
     def train(self, feat, target, contact, epochs=30, batch=4096, lr=3e-3, seed=0, log=None):
-        """Adam on host arrays (features n x 48, normalised log-cost targets n, contact flags n). Returns the loss curve."""
+        """Adam on host arrays (features n x 48, normalised log-cost targets n, contact flags n), reshuffled every epoch,
+        full batches only. The data go to the device once; each epoch gathers them into a shuffled copy there
+        (rsim_permute_words) and the batches are views into it (rsim_words_view), so an epoch costs one permutation
+        upload and one read of its losses. Returns the loss curve (the mean row loss of each epoch)."""
         rng = np.random.default_rng(seed)
         n = feat.shape[0]
         batch = min(batch, n)
-        d_f, d_t, d_c = (self.dev.alloc_words(batch * FEATURES), self.dev.alloc_words(batch), self.dev.alloc_words(batch))
-        scratch, loss = self.dev.alloc_words(batch * ACT_WORDS), self.dev.alloc_words(batch)
+        nb = n // batch if batch else 0
+        if nb == 0:
+            return []
+        dev = self.dev
+        widths = (FEATURES, 1, 1)
+        src = [dev.upload_words(np.ascontiguousarray(x, np.float32).reshape(-1).view(np.uint32)) for x in (feat, target, contact)]
+        shuffled = [dev.alloc_words(n * w) for w in widths]
+        perm = dev.alloc_words(n)
+        scratch, loss = dev.alloc_words(batch * ACT_WORDS), dev.alloc_words(nb * batch)
+        views = [[buf.view(b * batch * w, batch * w) for b in range(nb)] for buf, w in zip(shuffled, widths)]
+        loss_views = [loss.view(b * batch, batch) for b in range(nb)]
         curve = []
         for ep in range(epochs):
-            order = rng.permutation(n)
-            tot, cnt = 0.0, 0
-            for s in range(0, n - batch + 1, batch):
-                idx = order[s:s + batch]
-                self.dev.write_words(d_f, np.ascontiguousarray(feat[idx], np.float32).view(np.uint32))
-                self.dev.write_words(d_t, np.ascontiguousarray(target[idx], np.float32).view(np.uint32))
-                self.dev.write_words(d_c, np.ascontiguousarray(contact[idx], np.float32).view(np.uint32))
-                self.gradients(d_f, d_t, d_c, batch, scratch, loss)
+            order = np.ascontiguousarray(rng.permutation(n), np.uint32)
+            _check(lib().rsim_set_permutation(dev.handle, _ptr(order), n, perm.handle), "rsim_set_permutation")
+            for x, y, w in zip(src, shuffled, widths):
+                _check(lib().rsim_permute_words(dev.handle, x.handle, perm.handle, n, w, 0, y.handle), "rsim_permute_words")
+            for b in range(nb):
+                self.gradients(views[0][b], views[1][b], views[2][b], batch, scratch, loss_views[b])
                 self.step += 1
-                _check(lib().rsim_cost_adam(self.dev.handle, self.params.handle, self.m.handle, self.v.handle, self.grad.handle,
+                _check(lib().rsim_cost_adam(dev.handle, self.params.handle, self.m.handle, self.v.handle, self.grad.handle,
                                             lr, self.step), "rsim_cost_adam")
-                tot += float(self.dev.read_words(loss).view(np.float32).sum())
-                cnt += batch
-            curve.append(tot / max(cnt, 1))
+            curve.append(float(dev.read_words(loss).view(np.float32).astype(np.float64).sum()) / (nb * batch))
             if log:
                 log("epoch %d loss %.4f" % (ep, curve[-1]))
         return curve

@@ -304,3 +304,55 @@ def test_trained_model_predicts_cost_better_than_untrained(rsim, world, device, 
     after = corr(model)
     print("[case] held-out pearson: untrained %.3f trained %.3f, loss %.3f -> %.3f" % (before, after, curve[0], curve[-1]))
     assert after > 0.35 and after > before + 0.2, (before, after)
+
+
+@pytest.mark.gpu
+def test_device_batched_training_equals_host_batches(rsim, device):
+    """CostModel.train shuffles on the device and trains on views into the shuffled copy; the same epochs with the
+    batches gathered on the host and uploaded one by one give bit-identical parameters."""
+    note("training on device-side batches (rsim_words_view) vs host-gathered batches")
+    rng = np.random.default_rng(31)
+    n, batch, epochs = 5000, 512, 2  # 9 full batches per epoch, the remainder dropped
+    feat = rng.standard_normal((n, cm.FEATURES)).astype(np.float32)
+    target = rng.standard_normal(n).astype(np.float32)
+    contact = (rng.random(n) < 0.3).astype(np.float32)
+    a = cm.CostModel(device, seed=4)
+    a.train(feat, target, contact, epochs=epochs, batch=batch, lr=3e-3, seed=9)
+    b = cm.CostModel(device, seed=4)
+    order_rng = np.random.default_rng(9)
+    d_f, d_t, d_c = device.alloc_words(batch * cm.FEATURES), device.alloc_words(batch), device.alloc_words(batch)
+    scratch, loss = device.alloc_words(batch * cm.ACT_WORDS), device.alloc_words(batch)
+    for _ in range(epochs):
+        order = order_rng.permutation(n)
+        for s in range(0, n - batch + 1, batch):
+            idx = order[s:s + batch]
+            device.write_words(d_f, np.ascontiguousarray(feat[idx]).view(np.uint32).ravel())
+            device.write_words(d_t, np.ascontiguousarray(target[idx]).view(np.uint32))
+            device.write_words(d_c, np.ascontiguousarray(contact[idx]).view(np.uint32))
+            b.gradients(d_f, d_t, d_c, batch, scratch, loss)
+            b.step += 1
+            rsim._check(rsim.lib().rsim_cost_adam(device.handle, b.params.handle, b.m.handle, b.v.handle, b.grad.handle, 3e-3,
+                                                  b.step), "rsim_cost_adam")
+    device.sync()
+    assert a.step == b.step == epochs * (n // batch)
+    assert np.array_equal(a.host_params().view(np.uint32), b.host_params().view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_words_view(rsim, device):
+    note("rsim_words_view: a range of a buffer as a buffer of its own")
+    base = device.upload_words(np.arange(100, dtype=np.uint32))
+    v = base.view(10, 20)
+    assert v.n == 20 and np.array_equal(device.read_words(v), np.arange(10, 30, dtype=np.uint32))
+    device.write_words(v, np.full(20, 7, np.uint32))
+    got = device.read_words(base)
+    assert np.array_equal(got[10:30], np.full(20, 7)) and np.array_equal(got[:10], np.arange(10)) and got[30] == 30
+    perm = device.upload_words(np.arange(5, dtype=np.uint32)[::-1].copy())
+    dst = base.view(50, 5)  # held: a view's handle is freed with its Python object
+    rsim._check(rsim.lib().rsim_permute_words(device.handle, base.handle, perm.handle, 5, 1, 0, dst.handle),
+                "rsim_permute_words")
+    assert np.array_equal(device.read_words(base)[50:55], np.array([4, 3, 2, 1, 0], np.uint32))
+    del v
+    assert np.array_equal(device.read_words(base)[:10], np.arange(10))  # freeing a view leaves the memory alone
+    with pytest.raises(rsim.RsimError):
+        base.view(90, 11)
