@@ -257,7 +257,7 @@ def scale_leg(rsim, dev, decisions, block, threads, chunk=50):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)  # the contact-heavy configuration's 200 decisions
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--arenas", type=int, default=ARENAS)
