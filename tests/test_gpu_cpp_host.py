@@ -32,3 +32,35 @@ def test_kickoff_example(rsim, oracle):
         oracle.sim8(s, w, ctl)
     ball = s[0, 4:7].view(np.float32)
     assert ["%.1f" % v for v in ball] == ["%.1f" % v for v in got]
+
+
+def test_kickoff_example_from_v3_dumps(rsim, oracle, tmp_path):
+    """The reference's path in C++: the world and the kickoff handed over as V3 dumps (written here by the wire format's
+    writer), stepped on the GPU, arena 0 handed back as a V3 dump; it parses to the oracle's state after the same
+    decisions (every word but the last tick's events, which V3 does not export)."""
+    from rsim import v3
+    print("[case] C++ kickoff example from V3 dumps, 256 arenas x 6 decisions", flush=True)
+    w = rsim.build_world(0, 0)
+    seed = 0
+    while True:
+        s = rsim.sample_arenas(w, rsim.SCEN_KICKOFF, seed, 1)
+        if s[0, 29:30].view(np.float32)[0] == 0.0:
+            break
+        seed += 1
+    tpl = v3.Template()
+    tpl.config_hash = 0x5EED
+    tpl.num_handles = tpl.next_new_handle = 24
+    st, ar, out_path = tmp_path / "static.bin", tmp_path / "arena.bin", tmp_path / "out.bin"
+    st.write_bytes(v3.write_static(w))
+    ar.write_bytes(v3.write_arena(s[0], w, tpl))
+    out = subprocess.run([EXE, "256", "6", "--static", str(st), "--arena", str(ar), "--out", str(out_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got, t = v3.parse_arena(out_path.read_bytes(), w)
+    assert t.config_hash == 0x5EED
+    drive = np.array([struct.unpack("<I", struct.pack("<f", 1.0))[0], 0, 0, 0, 0, 0, 1, 0] * 2, np.uint32)
+    ref = s.copy()
+    for _ in range(7):  # warm-up decision + 6 timed
+        oracle.sim8(ref, w, drive.reshape(1, 16))
+    diff = np.flatnonzero(got[:2567] != ref[0, :2567])
+    assert set(diff.tolist()) <= {2566}, diff[:8]
