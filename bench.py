@@ -12,8 +12,9 @@ Workload: the contact-heavy mix (40% ball pinched against a wall or fillet with 
 
 Prints one JSON line. `roofline` prices the sim8 kernel against HBM at 20,544.5 algorithmic bytes per arena-tick (the
 2,567 persistent state words read and written once per tick, plus controls and result amortised over the decision).
-`cpu_baseline` is the CPU build of the same tick (the test oracle, g++ -O2, strict IEEE) on the host's cores over the
-first arenas of the same run; its outputs are word-compared with the GPU's (`word_diffs`).
+`cpu_baseline` is the reference's own CPU build of the path (its phase.cpp compiled in place, when built with the tree;
+otherwise the test oracle, bit-identical to it) on the host's cores over the arenas of the same run; its outputs are
+word-compared with the GPU's (`word_diffs`).
 """
 import argparse
 import json
@@ -42,20 +43,26 @@ def host_threads():
 
 
 def cpu_baseline_leg(world, states0, controls, gpu_states, gpu_results, sample, threads):
-    """The CPU build over arenas [0, sample) of the same run: timing plus a word compare with the GPU's outputs."""
+    """The CPU path over arenas [0, sample) of the same run, timed, and word-compared with the GPU's outputs. It is the
+    reference's own C++ build (crates/rocketsim-cuda/csrc/phase.cpp compiled in place into oracle/_ref, driven over host
+    threads) when that library was built with the tree, otherwise the oracle (the same tick built from this project's
+    core, bit-identical to the reference build by the tests)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    if not os.path.exists(orc.ORACLE_LIB):
-        orc.build(ref=False)
-    o = orc.Oracle()
+    if os.path.exists(orc.REF_LIB):
+        cpu, kind, what = orc.Reference(), "reference", "the reference's own C++ build (phase.cpp, g++ -O2 -ffp-contract=off)"
+    else:
+        if not os.path.exists(orc.ORACLE_LIB):
+            orc.build(ref=False)
+        cpu, kind, what = orc.Oracle(), "port", "CPU build of the same tick (g++ -O2 -ffp-contract=off)"
     s = np.ascontiguousarray(states0[:sample])
     ticks = 0
     diffs = 0
     t0 = time.perf_counter()
     res = []
     for c in controls:
-        r = o.sim8(s, world, np.ascontiguousarray(c[:sample]), threads=threads)
+        r = cpu.sim8(s, world, np.ascontiguousarray(c[:sample]), threads=threads)
         res.append(r)
     dt = time.perf_counter() - t0
     for k, r in enumerate(res):
@@ -63,9 +70,9 @@ def cpu_baseline_leg(world, states0, controls, gpu_states, gpu_results, sample, 
         ticks += int(np.where(sc > 0, sc, 8).sum())
         diffs += int((r != gpu_results[k][:sample]).sum())
     diffs += int((s[:, :2567] != gpu_states[:sample, :2567]).sum())
-    return {"value": ticks / dt, "unit": "arena-ticks/s", "cores": threads, "kind": "port",
-            "sample": "arenas 0..%d of the same run over all %d timed decisions (%d arena-ticks), %d host threads, "
-                      "CPU build of the same tick (g++ -O2 -ffp-contract=off)" % (sample - 1, len(controls), ticks, threads),
+    return {"value": ticks / dt, "unit": "arena-ticks/s", "cores": threads, "kind": kind,
+            "sample": "arenas 0..%d of the same run over all %d timed decisions (%d arena-ticks), %d host threads, %s"
+                      % (sample - 1, len(controls), ticks, threads, what),
             "seconds": dt}, diffs
 
 

@@ -207,6 +207,7 @@ class Reference:
         L.rscuda_phase.argtypes = [ctypes.c_int, _vp, _vp]
         L.rscuda_step_tick.argtypes = [_vp, _vp]
         L.rscuda_step_ticks.argtypes = [_vp, _vp, _vp, _u32, ctypes.POINTER(_u32), _vp]
+        L.refb_sim8_batch.argtypes = [_vp, _vp, _vp, _vp, _u32, _u32]
         self.L = L
 
     def phase(self, k, state, world):
@@ -220,6 +221,16 @@ class Reference:
         st = self.L.rscuda_step_ticks(_p(state), _p(world), _p(controls), n, ctypes.byref(sc),
                                       _p(trace) if trace is not None else None)
         return st, sc.value
+
+    def sim8(self, states, world, controls, threads=1):
+        """One decision of every arena of states (uint32[n, 2784], in place) by the reference's own CPU build, over
+        `threads` host threads; returns the sim8 result words."""
+        n = states.shape[0]
+        res = np.zeros(max(n, 1), np.uint32)
+        assert states.flags.c_contiguous and controls.flags.c_contiguous and controls.shape[0] >= n
+        if self.L.refb_sim8_batch(_p(states), _p(world), _p(controls), _p(res), n, threads) != 0:
+            raise RuntimeError("the reference's ArenaState is not 2,784 words")
+        return res[:n]
 
 
 LIBM_FUNCS = ("sin", "cos", "tan", "atan2", "pow", "atan2_64", "hypot64")  # bit f of a libm mask

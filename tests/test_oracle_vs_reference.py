@@ -62,3 +62,21 @@ def test_phases_vs_reference(oracle, ref):
     s = states[0].copy()
     assert oracle.phase(10, s, world) == ref.phase(10, s.copy(), world) == -2
     assert oracle.phase(-1, s, world) == -2
+
+
+def test_threaded_reference_batch(rsim, oracle, ref):
+    """The reference's decision over host threads (oracle/ref_batch.cpp, the bench's CPU baseline) gives the oracle's
+    result words and states, and the same as the reference called arena by arena."""
+    w = rsim.build_world(0, 0)
+    st = rsim.sample_arenas(w, rsim.SCEN_CONTACT, 515, 96)
+    a, b, c = st.copy(), st.copy(), st.copy()
+    for d in range(4):
+        ctl = rsim.sample_controls(516, d, 96)
+        ra = ref.sim8(a, w, ctl, threads=5)
+        rb = oracle.sim8(b, w, ctl, threads=3)
+        rc = []
+        for i in range(96):
+            status, scored = ref.step_ticks(c[i], w, ctl[i], 8)
+            rc.append(orc.result_word(status, scored, c[i]))
+        assert np.array_equal(ra, rb) and np.array_equal(ra, np.array(rc, np.uint32)), d
+        assert np.array_equal(a, b) and np.array_equal(a, c), d
