@@ -1,5 +1,6 @@
 """Parity of the gfx950 kernels with the CPU oracle, through librsim.so's C ABI: every TICK_WORDS word of every arena and
 every result word must be identical (the oracle is itself pinned to the reference's C++ build by the golden vectors)."""
+import os
 import sys
 
 import numpy as np
@@ -378,3 +379,21 @@ def test_pooled_walks_with_mixed_world_layouts(rsim, oracle, device):
         assert np.array_equal(oracle.sim8_in(cpu, worlds, variant, ctl[k], threads=4), got_r[k]), "decision %d" % k
     fd = first_diff(got[:, :TICK_WORDS], cpu[:, :TICK_WORDS])
     assert fd is None, "arena %d word %d" % fd
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                                                    "libref.so")), reason="reference build (oracle/_ref) not available")
+def test_against_the_reference_build(rsim, world, device, dev_world):
+    """The GPU against the reference's own C++ build directly (phase.cpp compiled in place, threaded by
+    oracle/ref_batch.cpp): 8,192 arenas of the contact and mixed workloads, 10 decisions, every result word and
+    persistent state word."""
+    import oracle as orc
+    ref = orc.Reference()
+    for scen in (rsim.SCEN_CONTACT, rsim.SCEN_MIXED):
+        note("GPU vs the reference build, scenario %d, 8192 arenas x 10 decisions" % scen)
+        states = rsim.sample_arenas(world, scen, 6060 + scen, 8192)
+        controls = [rsim.sample_controls(6070 + scen, d, 8192) for d in range(10)]
+        gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, controls)
+        cs = states.copy()
+        cr = [ref.sim8(cs, world, c, threads=8) for c in controls]
+        assert_same(gs, gr, cs, cr, "reference build, scenario %d" % scen)
