@@ -52,7 +52,8 @@ RS_FN void apply_damping(Body& b, const BodyConst& c, float dt) {  // :61-70: th
     b.lin_vel *= b.cached_linear_damping;
   }
 }
-RS_FN Aff predict_trans(const Body& b, float dt) {  // :73-82
+// inlined everywhere: a call would save and restore the caller's registers through private memory
+RS_INLINE Aff predict_trans(const Body& b, float dt) {  // :73-82
   Aff t = b.world;
   Quat q = b.world_quat;
   if ((b.collision_flags & k::CF_NO_ANGULAR_MOTION) != 0) t.t = t.t + b.lin_vel * dt;

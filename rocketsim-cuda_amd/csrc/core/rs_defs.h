@@ -16,11 +16,13 @@
 #define RS_HD __host__ __device__
 #define RS_FN __host__ __device__ inline
 #define RS_NOINL __host__ __device__ __attribute__((noinline)) inline
+#define RS_INLINE __host__ __device__ __attribute__((always_inline)) inline
 #define RS_TABLE static __device__ __constant__ const
 #else
 #define RS_HD
 #define RS_FN inline
 #define RS_NOINL inline
+#define RS_INLINE inline
 #define RS_TABLE static const
 #endif
 

@@ -155,7 +155,7 @@ RS_FN int clip_halfspace(const V3* in, int n, V3* out, int axis, float bound, bo
   }
   return m;
 }
-RS_NOINL bool clipped_centroid(V3 half, const V3 q[3], V3* res) {  // :576
+RS_INLINE bool clipped_centroid(V3 half, const V3 q[3], V3* res) {  // :576
   RS_PRIVATE(q); RS_PRIVATE(res);  // locals of the caller
   V3 poly[9], tmp[9];
   for (int i = 0; i < 9; ++i) {
@@ -283,7 +283,7 @@ RS_FN bool clip_segment_box(V3 a, V3 b, V3 half, V3* o0, V3* o1) {  // :783
   *o1 = a + d * t1;
   return true;
 }
-RS_NOINL bool pen_witness(V3 half, const V3 q[3], SatKind kind, V3 n, V3* res) {  // :735
+RS_INLINE bool pen_witness(V3 half, const V3 q[3], SatKind kind, V3 n, V3* res) {  // :735
   RS_PRIVATE(q); RS_PRIVATE(res);  // locals of the caller
   if (kind.tag == SAT_TRI_FACE) {
     float pd = dot(n, q[0]);
@@ -397,7 +397,7 @@ RS_FN void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri, V3
   *o_box = b0 + d2 * t;
 }
 // :960 the 47 candidate pairs: 3 clamped vertices, 8 box corners against the triangle, 3 x 12 edge pairs
-RS_NOINL void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr) {
+RS_INLINE void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr) {
   RS_PRIVATE(q); RS_PRIVATE(edges); RS_PRIVATE(pr);  // locals of the caller
   int n = 0;
   for (int i = 0; i < 3; ++i) {
@@ -433,7 +433,7 @@ RS_FN SatPair first_min_pair(const SatPair* pr) {  // strict <: the first pair w
   return best;
 }
 // :1096 flat_tie_segment: the triangle-side endpoints of a flat contact interval, or false
-RS_NOINL bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
+RS_INLINE bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
   RS_PRIVATE(pr); RS_PRIVATE(r0); RS_PRIVATE(r1);  // locals of the caller
   float link = 0.5f * max_elem(half);
   if (link <= 0.0f || !ffinite(link)) return false;
@@ -498,6 +498,9 @@ RS_NOINL bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
   return true;
 }
 // :52 box_triangle_sat: q is the triangle in the box frame, half the unmargined half extents. True for V3's Some.
+// One call per test; its four stages (the pair enumeration, the flat-tie segment, the penetration witness and the clipped
+// centroid) are inlined into it, each having this single caller: as calls they saved and restored the caller's
+// registers through private memory on every test.
 RS_NOINL bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
   RS_PRIVATE(&box_trans); RS_PRIVATE(q); RS_PRIVATE(out);  // locals of the caller
   V3 tmin = vmin(vmin(q[0], q[1]), q[2]), tmax = vmax(vmax(q[0], q[1]), q[2]);

@@ -55,7 +55,7 @@ RS_FN void phase_p6(ArenaState& s, const StaticData& w) {
 }
 
 // ---- P7 (arena_base.cuh:43-205, ball_base.cuh:22-48) ------------------------------------------------------------------------------
-RS_FN bool within_rect_cone(V3 fwd, V3 right, V3 up, V3 delta, bool reverse, float yaw_deg, float pitch_deg) {
+RS_INLINE bool within_rect_cone(V3 fwd, V3 right, V3 up, V3 delta, bool reverse, float yaw_deg, float pitch_deg) {
   V3 f = reverse ? -fwd : fwd, d;
   if (!try_normalize(delta, &d)) return false;
   float fd = dot(d, f);
