@@ -676,11 +676,17 @@ RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci) {  // car/ba
 }
 
 // P1 (arena_base_p1.cuh:11-20): the z-damp prepass for every car first, then each car's pre-tick; CarLanded events.
+// A car's pre-tick as a function of its own, called once per car: the boundary lets the register allocator treat the
+// car's update apart from the rest of the phase (measured faster than inlining it). Its state and world pointers carry
+// their address spaces, so its accesses compile to scratch_ and global_ instructions.
+RS_NOINL void car_pre_tick_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp, int ci) {
+  car_pre_tick(*(ArenaState*)sp, *(const StaticData*)wp, ci);
+}
 RS_FN void phase_p1(ArenaState& s, const StaticData& w) {
   for (int ci = 0; ci < NUM_CARS; ++ci) car_flip_zdamp(s, w, ci);
   for (int ci = 0; ci < NUM_CARS; ++ci) {
     uint32_t was = s.cars[ci].is_on_ground;
-    car_pre_tick(s, w, ci);
+    car_pre_tick_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w, ci);
     if (!was && s.cars[ci].is_on_ground) s.events |= 1u << (EV_LANDED + ci);
   }
 }
