@@ -436,14 +436,15 @@ RS_FN void mesh_rays_leaves(const StaticData& w, const MeshInfo& mesh, BodyRays&
   if (count == 0) return;
   float org[3][4], inv[3][4];
   packet_pos_dir(rp, org, inv);
-  // the next leaf is loaded before this one's tests (loads only: the same sequence of tests)
+  // the next leaf is loaded before this one's tests, and a leaf's triangle before its slab test, so the triangle's load
+  // overlaps the test instead of following it (loads only: the same sequence of tests)
   Leaf next = w.leaves[mesh.leaf_first + leaves[0]];
   for (uint32_t k = 0; k < count; ++k) {
     const Leaf leaf = next;
     if (k + 1 < count) next = w.leaves[mesh.leaf_first + leaves[k + 1]];
+    const Tri tri = w.tris[mesh.tri_first + leaf.tri];
     uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
     if (mask != 0) {
-      const Tri& tri = w.tris[mesh.tri_first + leaf.tri];
       for (int i = 0; i < 4; ++i)
         if (mask & (1u << i)) ray_triangle(br, tri, &rp.lambda_max[i], i);
     }
