@@ -132,58 +132,79 @@ __device__ inline bool trav_pooled(const StaticData& w, TravBatch& tb) {
   wave_lds_sync();
   uint32_t cur = 0;
   for (uint32_t depth = 0;; ++depth) {  // one BVH level per pass; every lane sees the same counts
-    const uint32_t n = *wave_word(PT_CTRL_W + cur);
-    if (n == 0) break;
+    const uint32_t n_all = *wave_word(PT_CTRL_W + cur);
+    if (n_all == 0) break;
+    // a level that overflowed the queue is flagged already (its walk is then redone per lane); only its first PT_Q
+    // items were stored, so only those are read
+    const uint32_t n = n_all < PT_Q ? n_all : PT_Q;
     if (depth >= PT_DEPTH) {
       if (rank == 0) *wave_word(PT_CTRL_W + 3) = 1u;
       break;
     }
-    for (uint32_t base = 0; base < n; base += nact) {
-      const uint32_t i = base + rank;
-      if (i >= n) continue;
-      const uint32_t w0 = *pt_q(cur, i, 0), key = *pt_q(cur, i, 1);
+    // each lane's items i = rank, rank + nact, ...; the next item's record and node are loaded before this item's
+    // children are queued, so the node's load overlaps the queueing (the queues' order is free: leaves are ranked by key)
+    uint32_t i = rank, w0 = 0u, key = 0u, ok = 0u;
+    Aabb box = {};
+    WideNode nd = {};
+    auto fetch = [&](uint32_t it) {
+      w0 = *pt_q(cur, it, 0);
+      key = *pt_q(cur, it, 1);
       const uint32_t ref = w0 >> 20, node = w0 & 0xFFFFFu;
-      if (ref >= PT_JOBS || node >= *wave_word(PT_JOB_W + PT_REC * ref + 8)) {  // never for a well-formed BVH
+      ok = (ref < PT_JOBS && node < *wave_word(PT_JOB_W + PT_REC * ref + 8)) ? 1u : 0u;
+      if (!ok) {  // never for a well-formed BVH
         *wave_word(PT_CTRL_W + 3) = 1u;
-        continue;
+        return;
       }
       const uint32_t rec = PT_JOB_W + PT_REC * ref;
-      Aabb box;
       box.min = v3(__builtin_bit_cast(float, *wave_word(rec)), __builtin_bit_cast(float, *wave_word(rec + 1)),
                    __builtin_bit_cast(float, *wave_word(rec + 2)));
       box.max = v3(__builtin_bit_cast(float, *wave_word(rec + 3)), __builtin_bit_cast(float, *wave_word(rec + 4)),
                    __builtin_bit_cast(float, *wave_word(rec + 5)));
       const WideNode* nodes =
           (const WideNode*)(uintptr_t)(((uint64_t)*wave_word(rec + 7) << 32) | (uint64_t)*wave_word(rec + 6));
-      const WideNode& nd = nodes[node];
-      const uint32_t mask = node_mask(nd, box);
-      for (uint32_t l = 0; l < nd.child_count && l < 4u; ++l) {
-        if (!((mask >> l) & 1u)) continue;
-        const uint32_t ch = nd.children[l], k2 = key | (l << (28u - 2u * depth));
-        const uint32_t idx = ch & ~BVH_LEAF;
-        if (idx >= (1u << 20)) {
-          *wave_word(PT_CTRL_W + 3) = 1u;
-          continue;
-        }
-        if (ch & BVH_LEAF) {
-          const uint32_t slot = atomicAdd(wave_word(PT_CTRL_W + 2), 1u);
-          if (slot < PT_L) {
-            *wave_word(PT_L_W + 2 * slot) = (ref << 20) | idx;
-            *wave_word(PT_L_W + 2 * slot + 1) = k2;
-            atomicAdd(wave_word(PT_CNT_W + ref), 1u);
-          } else {
+      nd = nodes[node];
+    };
+    if (i < n) fetch(i);
+    while (i < n) {
+      const uint32_t c_ok = ok, c_key = key, c_ref = w0 >> 20;
+      uint32_t mask = 0u, children[4] = {0u, 0u, 0u, 0u}, child_count = 0u;
+      if (c_ok) {
+        mask = node_mask(nd, box);
+        for (int l = 0; l < 4; ++l) children[l] = nd.children[l];
+        child_count = nd.child_count;
+      }
+      const uint32_t next = i + nact;
+      if (next < n) fetch(next);
+      if (c_ok) {
+        for (uint32_t l = 0; l < child_count && l < 4u; ++l) {
+          if (!((mask >> l) & 1u)) continue;
+          const uint32_t ch = children[l], k2 = c_key | (l << (28u - 2u * depth));
+          const uint32_t idx = ch & ~BVH_LEAF;
+          if (idx >= (1u << 20)) {
             *wave_word(PT_CTRL_W + 3) = 1u;
+            continue;
           }
-        } else {
-          const uint32_t slot = atomicAdd(wave_word(PT_CTRL_W + (cur ^ 1u)), 1u);
-          if (slot < PT_Q) {
-            *pt_q(cur ^ 1u, slot, 0) = (ref << 20) | idx;
-            *pt_q(cur ^ 1u, slot, 1) = k2;
+          if (ch & BVH_LEAF) {
+            const uint32_t slot = atomicAdd(wave_word(PT_CTRL_W + 2), 1u);
+            if (slot < PT_L) {
+              *wave_word(PT_L_W + 2 * slot) = (c_ref << 20) | idx;
+              *wave_word(PT_L_W + 2 * slot + 1) = k2;
+              atomicAdd(wave_word(PT_CNT_W + c_ref), 1u);
+            } else {
+              *wave_word(PT_CTRL_W + 3) = 1u;
+            }
           } else {
-            *wave_word(PT_CTRL_W + 3) = 1u;
+            const uint32_t slot = atomicAdd(wave_word(PT_CTRL_W + (cur ^ 1u)), 1u);
+            if (slot < PT_Q) {
+              *pt_q(cur ^ 1u, slot, 0) = (c_ref << 20) | idx;
+              *pt_q(cur ^ 1u, slot, 1) = k2;
+            } else {
+              *wave_word(PT_CTRL_W + 3) = 1u;
+            }
           }
         }
       }
+      i = next;
     }
     wave_lds_sync();
     if (rank == 0) *wave_word(PT_CTRL_W + cur) = 0u;  // this level's frontier is consumed
