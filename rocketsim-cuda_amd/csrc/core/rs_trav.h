@@ -237,6 +237,8 @@ __device__ inline bool trav_pooled(const StaticData& w, TravBatch& tb) {
     const uint32_t w0 = *pt_q(0, i, 0), key = *pt_q(0, i, 1), ref = w0 >> 20;
     const uint32_t s0 = *wave_word(PT_START_W + ref), cnt = *wave_word(PT_CNT_W + ref);
     uint32_t r = 0;
+    // unrolled so that several of the segment's key loads are in flight at once
+#pragma unroll 4
     for (uint32_t j = s0; j < s0 + cnt; ++j) r += *pt_q(0, j, 1) < key ? 1u : 0u;
     *pt_q(1, s0 + r, 0) = w0 & 0xFFFFFu;
   }
