@@ -118,9 +118,10 @@ def train_cost_model(rsim, dev, world_host, dw):
 
 def compare_orders(rsim, dev, dw, model, states_host, controls, block, plain=None):
     """K decisions of one batch in the plain order (per-launch sim8) and regrouped by predicted cost before every decision
-    ('sorted': cost-homogeneous waves, most expensive first; 'balanced': each wave dealt one arena of every cost rank).
-    Device time over all K decisions (regrouping included), and whether every result word and final state equals the
-    plain run's."""
+    ('sorted': cost-homogeneous waves, most expensive first; 'balanced': each wave dealt one arena of every cost rank),
+    and balanced regrouping before every 8th or 32nd decision only ('balanced_every8', 'balanced_every32': the
+    regrouping's cost amortised, the order aging in between). Device time over all K decisions (regrouping included), and whether every result word and
+    final state equals the plain run's."""
     import rsim.cost as cm
 
     n, K = states_host.shape[0], len(controls)
@@ -144,13 +145,15 @@ def compare_orders(rsim, dev, dw, model, states_host, controls, block, plain=Non
         results, final, ms = plain
         stepped = stepped_ticks(results)
         out["plain"] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K, "note": "the timed run"}
-    for mode in ("sorted", "balanced"):
+    for name, mode, every in (("sorted", "sorted", 1), ("balanced", "balanced", 1), ("balanced_every8", "balanced", 8),
+                              ("balanced_every32", "balanced", 32)):
         dev.copy_states(d_work, d_init)
         g = cm.Regrouper(dev, model, dw, d_work, mode, block)
         dev.sync()
         dev.timer_begin()
         for k in range(K):
-            g.regroup()
+            if k % every == 0:
+                g.regroup()
             g.step(d_ctl[k], d_res[k])
         ms = dev.timer_end()
         same = all(np.array_equal(dev.read_words(d_res[k]), results[k]) for k in range(K))
@@ -158,10 +161,10 @@ def compare_orders(rsim, dev, dw, model, states_host, controls, block, plain=Non
         same = same and bool(np.array_equal(fin[:, :2567], final[:, :2567]))
         dev.sync()
         dev.timer_begin()
-        for k in range(K):
+        for k in range(0, K, every):
             g.regroup()
         reg_ms = dev.timer_end()
-        out[mode] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K,
+        out[name] = {"arena_ticks_per_s": stepped / (ms / 1000.0), "ms_per_decision": ms / K,
                      "regroup_ms_per_decision": reg_ms / K, "identical_to_plain": same,
                      "vs_plain": out["plain"]["ms_per_decision"] / (ms / K)}
         del g
