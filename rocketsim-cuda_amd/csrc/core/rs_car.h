@@ -7,10 +7,13 @@
 namespace rs {
 
 // ---- the wheel (wheel_info.cuh) ---------------------------------------------------------------------------------------------
+RS_FN void wheel_ray_ends(const WheelConst& wc, const Aff& chassis, V3* src, V3* dst) {  // :14-18 (the ray's ends)
+  *src = xform_point(chassis, wc.connection_point);
+  *dst = *src - chassis.m.c2 * wc.ray_length;
+}
 RS_FN void wheel_prepare(Wheel& wh, const WheelConst& wc, const Aff& chassis, V3* src, V3* dst) {  // :14-18
-  wh.hard_point = xform_point(chassis, wc.connection_point);
-  *dst = wh.hard_point - chassis.m.c2 * wc.ray_length;
-  *src = wh.hard_point;
+  wheel_ray_ends(wc, chassis, src, dst);
+  wh.hard_point = *src;
 }
 RS_FN void wheel_reset(Wheel& wh) {
   wh.extra_pushback = 0.0f;
@@ -175,7 +178,8 @@ RS_FN void wheel_dynamic_effects(ArenaState& s, const StaticData& w, const Wheel
   add_impulse(victim, w.bodies[ri.ground_body], IMP_LINEAR_REL_POS, -ri.contact_normal * full, ri.contact_point - victim.world.t,
               true, false);
 }
-RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, float handbrake, float throttle, bool three_wheels) {
+RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, float handbrake, float throttle, bool three_wheels,
+                          const WheelRayPrep& prep) {
   Car& car = s.cars[ci];
   const CarConst& cc = w.cars[ci];
   const uint32_t chassis_body = cc.body_idx;
@@ -187,7 +191,7 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
   V3 src[NUM_WHEELS], dst[NUM_WHEELS];
   for (int i = 0; i < NUM_WHEELS; ++i) wheel_prepare(car.wheels[i], cc.wheels[i], ct, &src[i], &dst[i]);
   WheelHit hits[NUM_WHEELS];
-  cast_wheel_rays(s, w, src, dst, chassis_body, hits);
+  cast_wheel_rays(s, w, src, dst, chassis_body, prep, ci, hits);
   bool cache_some = false;  // the front wheels share one steered axle while their angle matches
   float cache_angle = 0.0f;
   V3 cache_axle = splat(0.0f);
@@ -625,7 +629,7 @@ RS_FN void car_flip_zdamp(ArenaState& s, const StaticData& w, int ci) {  // appl
   Body& rb = s.bodies[dyn_slot(w, w.cars[ci].body_idx)];
   if (rb.lin_vel.z < 0.0f || car.flip_time < k::FLIP_Z_DAMP_END) rb.lin_vel.z = rb.lin_vel.z * (1.0f - k::FLIP_Z_DAMP_120);
 }
-RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci) {  // car/base.rs pre_tick_update (car_pre_tick.cuh:638-703)
+RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci, const WheelRayPrep& prep) {  // car/base.rs pre_tick_update (car_pre_tick.cuh:638-703)
   Car& car = s.cars[ci];
   const CarConst& cc = w.cars[ci];
   Body& rb = s.bodies[dyn_slot(w, cc.body_idx)];
@@ -667,7 +671,7 @@ RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci) {  // car/ba
   car.world_contact_normal = Opt<V3>{};
   car_update_boost(s, w, ci, rb);
   float real_throttle = (car.controls.boost && car.boost > 0.0f) ? 1.0f : quant_throttle;
-  vehicle_update(s, w, ci, k::TICK_TIME, car.handbrake_val, real_throttle, cc.three_wheels != 0);
+  vehicle_update(s, w, ci, k::TICK_TIME, car.handbrake_val, real_throttle, cc.three_wheels != 0, prep);
   car_update_contact_state(s, w, ci);
   uint32_t any_world = 0;
   for (int i = 0; i < NUM_WHEELS; ++i)
@@ -679,14 +683,39 @@ RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci) {  // car/ba
 // A car's pre-tick as a function of its own, called once per car: the boundary lets the register allocator treat the
 // car's update apart from the rest of the phase (measured faster than inlining it). Its state and world pointers carry
 // their address spaces, so its accesses compile to scratch_ and global_ instructions.
-RS_NOINL void car_pre_tick_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp, int ci) {
-  car_pre_tick(*(ArenaState*)sp, *(const StaticData*)wp, ci);
+RS_NOINL void car_pre_tick_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp, int ci,
+                              const RS_PRIVATE_AS WheelRayPrep* pp) {
+  car_pre_tick(*(ArenaState*)sp, *(const StaticData*)wp, ci, *(const WheelRayPrep*)pp);
+}
+// Both cars' wheel-ray screens and mesh walks, before either car's pre-tick (exact: see WheelRayPrep). A demoed car casts
+// no rays this tick (its pre-tick respawns it at most); its share is empty.
+RS_FN void wheel_rays_prepare(const ArenaState& s, const StaticData& w, WheelRayPrep& prep) {
+  prep.tb.n_jobs = prep.tb.n_leaves = prep.tb.next = 0;
+  prep.tb.leaves = 1;
+  for (int ci = 0; ci < NUM_CARS; ++ci) {
+    prep.ncand[ci] = 0;
+    prep.first_job[ci] = prep.tb.n_jobs;
+    if (!s.cars[ci].is_demoed) {
+      const CarConst& cc = w.cars[ci];
+      const Aff ct = s.bodies[dyn_slot(w, cc.body_idx)].world;
+      V3 src[NUM_WHEELS], dst[NUM_WHEELS];
+      for (int i = 0; i < NUM_WHEELS; ++i) wheel_ray_ends(cc.wheels[i], ct, &src[i], &dst[i]);
+      ClosestHits hits;
+      wheel_hits_init(hits, src, dst, cc.body_idx);
+      prep.ncand[ci] = ray_screen(s, w, hits, prep.cand[ci]);
+      ray_jobs(s, w, hits, prep.cand[ci], prep.ncand[ci], prep.tb);
+    }
+    prep.end_job[ci] = prep.tb.n_jobs;
+  }
+  trav_run(w, prep.tb);
 }
 RS_FN void phase_p1(ArenaState& s, const StaticData& w) {
   for (int ci = 0; ci < NUM_CARS; ++ci) car_flip_zdamp(s, w, ci);
+  WheelRayPrep prep;
+  wheel_rays_prepare(s, w, prep);
   for (int ci = 0; ci < NUM_CARS; ++ci) {
     uint32_t was = s.cars[ci].is_on_ground;
-    car_pre_tick_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w, ci);
+    car_pre_tick_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w, ci, (const RS_PRIVATE_AS WheelRayPrep*)&prep);
     if (!was && s.cars[ci].is_on_ground) s.events |= 1u << (EV_LANDED + ci);
   }
 }

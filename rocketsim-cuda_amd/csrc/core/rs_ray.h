@@ -502,9 +502,9 @@ RS_FN void body_ray_setup(const ArenaState& s, const StaticData& w, uint32_t bod
 RS_FN bool rays_walk_mesh(uint32_t shape_kind) {
   return shape_kind != k::SHAPE_COMPOUND && shape_kind != k::SHAPE_SPHERE && shape_kind != k::SHAPE_PLANE;
 }
-// `tb` (may be null): mesh walks done beforehand, taken in order through *next_job (grid_ray_test collects them)
+// `tb` (may be null): mesh walks done beforehand, taken in order through *next_job up to end_job (ray_jobs collects them)
 RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, ClosestHits& h, const TravBatch* tb,
-                     uint32_t* next_job) {
+                     uint32_t* next_job, uint32_t end_job) {
   const BodyConst& bc = w.bodies[body];
   BodyRays br;
   RayPacket rp;
@@ -519,7 +519,7 @@ RS_FN void body_rays(const ArenaState& s, const StaticData& w, uint32_t body, Cl
     // every mesh record of the body, in record order, from its walked leaf list when the batch holds one
     for (int mi = next_mesh_of(w, body, 0); mi < NUM_MESHES; mi = next_mesh_of(w, body, mi + 1)) {
       const TravJob* jb = nullptr;
-      if (tb && *next_job < tb->n_jobs && tb->job[*next_job].body == body && tb->job[*next_job].mesh_body == (uint32_t)mi)
+      if (tb && *next_job < end_job && tb->job[*next_job].body == body && tb->job[*next_job].mesh_body == (uint32_t)mi)
         jb = &tb->job[(*next_job)++];
       if (jb && !jb->in_place) mesh_rays_leaves(w, w.meshes[mi], br, rp, tb->leaf_tri + jb->first, jb->count);
       else mesh_rays(w, w.meshes[mi], br, rp);
@@ -547,13 +547,16 @@ RS_FN bool ray_candidate_passes(const ProxyConst& p, const ClosestHits& hits) { 
   return hits.ignore_body != p.client_body && (p.filter_group & hits.mask) != 0 && (hits.group & p.filter_mask) != 0;
 }
 RS_FN void ray_candidate(const ArenaState& s, const StaticData& w, uint32_t h, ClosestHits& hits, const TravBatch* tb,
-                         uint32_t* next_job) {
+                         uint32_t* next_job, uint32_t end_job) {
   const ProxyConst& p = w.proxies[h];
-  if (ray_candidate_passes(p, hits)) body_rays(s, w, p.client_body, hits, tb, next_job);
+  if (ray_candidate_passes(p, hits)) body_rays(s, w, p.client_body, hits, tb, next_job, end_job);
 }
-// grid_broadphase.rs ray_test (raycaster.cuh:100-148): the cells of the four ray sources in ray order; in each cell its
-// static handles, then its dynamic ones; every handle at most once.
-RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& hits) {
+// grid_broadphase.rs ray_test (raycaster.cuh:100-148), in three steps. The screen: the cells of the four ray sources in
+// ray order; in each cell its static handles, then its dynamic ones; every handle at most once, kept if its box overlaps
+// the rays' box. The screen runs over all cells before any candidate's ray test: a ray test changes only the hits, which
+// the screen does not read, so this is the sequence of screening and testing entry by entry, with the screen's loads no
+// longer waiting behind the tests. Returns the number of candidates written to cand (at most 32).
+RS_FN int ray_screen(const ArenaState& s, const StaticData& w, const ClosestHits& hits, uint32_t* cand) {
   Aabb rb;
   rb.min = vmin(hits.from[0], hits.to[0]);
   rb.max = vmax(hits.from[0], hits.to[0]);
@@ -569,10 +572,7 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
     for (int q = 0; q < nc; ++q) found = found || cells[q] == c;
     if (!found) cells[nc++] = c;
   }
-  // the screen (first visit, box overlap) over all cells first, then the candidates' ray tests in the same order: a ray
-  // test changes only the hits, which the screen does not read, so this is the sequence of screening and testing entry by
-  // entry, with the screen's loads no longer waiting behind the tests
-  uint32_t seen = 0, cand[32];  // distinct handles: at most one per bit of `seen`
+  uint32_t seen = 0;  // distinct handles: at most one per bit of `seen`, so at most 32 candidates
   int ncand = 0;
   for (int ci = 0; ci < nc; ++ci) {
     uint32_t cell = cells[ci];
@@ -599,11 +599,13 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
     }
   }
-  // the candidates' mesh walks, in the order the tests below reach them, walked together first (on the GPU pooled across
-  // the wave's lanes); a walk's node test reads only the packet's box in the mesh frame, which the hits do not change
-  TravBatch tb;
-  tb.n_jobs = tb.n_leaves = tb.next = 0;
-  tb.leaves = 1;
+  return ncand;
+}
+// The candidates' mesh walks, in the order ray_tests reaches them, appended to tb (as many as fit); a walk's node test reads
+// only the packet's box in the mesh frame, which the hits do not change, so the walks can run before the tests (on the GPU
+// pooled across the wave's lanes).
+RS_FN void ray_jobs(const ArenaState& s, const StaticData& w, ClosestHits& hits, const uint32_t* cand, int ncand,
+                    TravBatch& tb) {
   for (int k = 0; k < ncand && tb.n_jobs < (uint32_t)TJ_MAX; ++k) {
     const ProxyConst& p = w.proxies[cand[k]];
     const uint32_t body = p.client_body;
@@ -619,9 +621,12 @@ RS_FN void grid_ray_test(const ArenaState& s, const StaticData& w, ClosestHits& 
       jb.box = rp.box;
     }
   }
-  trav_run(w, tb);
-  uint32_t next_job = 0;
-  for (int k = 0; k < ncand; ++k) ray_candidate(s, w, cand[k], hits, &tb, &next_job);
+}
+// The candidates' ray tests in screen order, mesh candidates taking their walked leaves from tb[first_job, end_job).
+RS_FN void ray_tests(const ArenaState& s, const StaticData& w, ClosestHits& hits, const uint32_t* cand, int ncand,
+                     const TravBatch& tb, uint32_t first_job, uint32_t end_job) {
+  uint32_t next_job = first_job;
+  for (int k = 0; k < ncand; ++k) ray_candidate(s, w, cand[k], hits, &tb, &next_job, end_job);
 }
 
 struct WheelHit {
@@ -629,14 +634,27 @@ struct WheelHit {
   V3 point, normal;
   uint32_t body;
 };
-// vehicle/raycaster.rs cast_rays (raycaster.cuh:153-176)
-RS_FN void cast_wheel_rays(const ArenaState& s, const StaticData& w, const V3* from, const V3* to, uint32_t ignore,
-                           WheelHit out[4]) {
-  ClosestHits hits;
+RS_FN void wheel_hits_init(ClosestHits& hits, const V3* from, const V3* to, uint32_t ignore) {
   hits_init(hits, from, to, ignore);
   hits.group |= k::FILTER_DROPSHOT_FLOOR;
   hits.min_hit_dist_sq = 0.116684f;  // WHEEL_RAY_MIN_HIT_DIST_SQ
-  grid_ray_test(s, w, hits);
+}
+// Both cars' wheel-ray queries of a tick, screened and walked together at the start of P1 (one pooled walk per tick). A
+// car's packets are fixed by its chassis transform and wheel geometry, its candidates by the grid and the dynamic proxies
+// and cells: P1 writes none of these before the car's rays are cast (a car's transform moves only when it respawns, and a
+// respawning car casts no rays that tick), so the screen and the walks done early are the ones done at the cast.
+struct WheelRayPrep {
+  TravBatch tb;
+  uint32_t cand[NUM_CARS][32];
+  int ncand[NUM_CARS];
+  uint32_t first_job[NUM_CARS], end_job[NUM_CARS];
+};
+// vehicle/raycaster.rs cast_rays (raycaster.cuh:153-176), the screen and walks taken from the car's share of `prep`
+RS_FN void cast_wheel_rays(const ArenaState& s, const StaticData& w, const V3* from, const V3* to, uint32_t ignore,
+                           const WheelRayPrep& prep, int ci, WheelHit out[4]) {
+  ClosestHits hits;
+  wheel_hits_init(hits, from, to, ignore);
+  ray_tests(s, w, hits, prep.cand[ci], prep.ncand[ci], prep.tb, prep.first_job[ci], prep.end_job[ci]);
   for (int i = 0; i < 4; ++i) {
     out[i].some = 0;
     out[i].point = splat(0.0f);
