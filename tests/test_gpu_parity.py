@@ -88,6 +88,25 @@ def test_scenarios(rsim, oracle, world, device, dev_world, scenario):
     assert_same(gs, gr, cs, cr, "scenario %d" % scenario)
 
 
+def test_respawn_beside_wheel_rays(rsim, oracle, world, device, dev_world):
+    """One car of each arena demoed with its respawn timer 1-12 ticks from zero, so it respawns (its body moves) in P1
+    of a tick in which the other car casts its wheel rays from the walks pooled at the start of that P1."""
+    from rsim import layout
+    note("respawn beside wheel rays, 2048 contact arenas x 3 decisions")
+    n = 2048
+    states = rsim.sample_arenas(world, rsim.SCEN_CONTACT, 4321, n)
+    cars = layout.arena_view(states)["cars"]
+    for a in range(n):
+        cars["is_demoed"][a, a % 2] = 1
+        cars["demo_respawn_timer"][a, a % 2] = np.float32((1 + a % 12) / 120.0)
+    assert int(layout.arena_view(states)["cars"]["is_demoed"].sum()) == n
+    ctl = [rsim.sample_controls(4322, d, n) for d in range(3)]
+    gs, gr, _ = gpu_decisions(rsim, device, dev_world, states, ctl)
+    cs, cr = cpu_decisions(oracle, world, states, ctl)
+    assert_same(gs, gr, cs, cr, "respawn")
+    assert not layout.arena_view(gs)["cars"]["is_demoed"].any(), "every demoed car respawned"
+
+
 @pytest.mark.parametrize("presets", [(1, 2), (3, 4), (5, 6), (6, 0)])
 def test_presets(rsim, oracle, device, presets):
     note("presets %s, contact mix, 512 arenas x 3 decisions" % (presets,))

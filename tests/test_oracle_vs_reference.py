@@ -80,3 +80,28 @@ def test_threaded_reference_batch(rsim, oracle, ref):
             rc.append(orc.result_word(status, scored, c[i]))
         assert np.array_equal(ra, rb) and np.array_equal(ra, np.array(rc, np.uint32)), d
         assert np.array_equal(a, b) and np.array_equal(a, c), d
+
+
+def test_respawn_beside_wheel_rays(rsim, oracle, ref):
+    """A car respawning in the same tick as the other car casts its wheel rays: one car of each arena is demoed with its
+    respawn timer a few ticks from zero (a different tick per arena), so the respawn (which moves that car's body) falls
+    inside P1 of one of the first decisions, and the whole states and result words must equal the reference's."""
+    from rsim import layout
+    w = rsim.build_world(0, 0)
+    n = 256
+    st = rsim.sample_arenas(w, rsim.SCEN_CONTACT, 4321, n)
+    cars = layout.arena_view(st)["cars"]
+    for a in range(n):
+        ci = a % 2
+        cars["is_demoed"][a, ci] = 1
+        cars["demo_respawn_timer"][a, ci] = np.float32((1 + a % 12) / 120.0)
+    assert int(layout.arena_view(st)["cars"]["is_demoed"].sum()) == n  # the view writes through to the states
+    a_, b_ = st.copy(), st.copy()
+    for d in range(3):
+        ctl = rsim.sample_controls(4322, d, n)
+        ra = ref.sim8(a_, w, ctl, threads=4)
+        rb = oracle.sim8(b_, w, ctl, threads=4)
+        assert np.array_equal(ra, rb), d
+        assert np.array_equal(a_, b_), d
+    after = layout.arena_view(a_)["cars"]["is_demoed"]
+    assert not after.any(), "every demoed car respawned"
