@@ -9,7 +9,11 @@
  *     reference's word format (cuda/sim/arena_state.cuh:38-342: ArenaState 11,136 B, StaticData 1,271,580 B, Controls
  *     32 B = throttle, steer, pitch, yaw, roll as f32, then jump, boost, handbrake as u32 0/1);
  *   - all device work of one rsim_device goes to its one stream, in call order: rsim_sim8 and rsim_run are asynchronous,
- *     everything that returns host data synchronises first. A device handle is not thread-safe.
+ *     everything that returns host data synchronises first. A device handle is not thread-safe;
+ *   - while the stream records a graph (rsim_capture_begin .. rsim_capture_end) every call that allocates, frees, copies
+ *     to or from the host, times or synchronises (rsim_alloc_*, rsim_upload_*, rsim_write_*, rsim_read_words,
+ *     rsim_download_*, rsim_*_free, rsim_sync, rsim_timer_*, the parity kernels, rsim_close) returns RSIM_ERR_INVALID
+ *     and changes nothing: allocate every buffer a captured sequence uses before the capture.
  * rsim_last_error() describes the last failure of the calling thread. */
 #ifndef RSIM_H
 #define RSIM_H
@@ -64,6 +68,9 @@ typedef struct rsim_words rsim_words;   /* a plain u32 device buffer (controls, 
 
 /* SimDevice::new (device.rs:53-67): opens GPU `ordinal`, creates its stream (default priority) and timing events. */
 int rsim_open(int ordinal, rsim_device** dev);
+/* Releases the device. Every world, state buffer, word buffer (views included) and graph made on it must be freed
+ * first: while any is open this returns RSIM_ERR_INVALID and the device stays usable (the reference's Rust ownership
+ * makes the order impossible to get wrong; a C caller gets the refusal instead of a freed device behind a live handle). */
 int rsim_close(rsim_device* dev);
 /* The device's HIP stream (a hipStream_t) and its ordinal, for callers that order their own work against it. */
 void* rsim_stream(rsim_device* dev);

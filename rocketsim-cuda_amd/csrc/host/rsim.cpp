@@ -25,6 +25,7 @@ struct rsim_device {
   hipEvent_t t0, t1;
   int32_t* status;  // per-arena status words of the parity kernels
   uint32_t status_cap;
+  uint64_t live;  // worlds, state and word buffers (views included) and graphs not yet freed: rsim_close refuses while > 0
 };
 struct rsim_world {
   rsim_device* dev;
@@ -70,6 +71,18 @@ int hip_fail(hipError_t e, const char* what) {
 int use(rsim_device* dev) {
   if (!dev) return fail(RSIM_ERR_INVALID, "null device");
   RSIM_HIP(hipSetDevice(dev->ordinal), "hipSetDevice");
+  return RSIM_OK;
+}
+// A call that allocates, frees, copies to or from the host or synchronises cannot run while the device's stream records a
+// graph: it would record a memset or a copy into the graph, or its synchronisation would end the capture with an error.
+// Such calls refuse (RSIM_ERR_INVALID) and leave everything as it was.
+int use_outside_capture(rsim_device* dev, const char* what) {
+  int rc = use(dev);
+  if (rc) return rc;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  RSIM_HIP(hipStreamIsCapturing(dev->stream, &st), "hipStreamIsCapturing");
+  if (st != hipStreamCaptureStatusNone)
+    return fail(RSIM_ERR_INVALID, (std::string(what) + ": not allowed while the device's stream records a graph").c_str());
   return RSIM_OK;
 }
 
@@ -162,8 +175,10 @@ int rsim_open(int ordinal, rsim_device** out) {
 }
 int rsim_close(rsim_device* dev) {
   if (!dev) return RSIM_OK;
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_close");
   if (rc) return rc;
+  if (dev->live)
+    return fail(RSIM_ERR_INVALID, "rsim_close: worlds, buffers or graphs of this device are still open (free them first)");
   (void)hipStreamSynchronize(dev->stream);
   if (dev->status) (void)hipFree(dev->status);
   (void)hipEventDestroy(dev->t0);
@@ -174,19 +189,19 @@ int rsim_close(rsim_device* dev) {
 }
 void* rsim_stream(rsim_device* dev) { return dev ? (void*)dev->stream : nullptr; }
 int rsim_sync(rsim_device* dev) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_sync");
   if (rc) return rc;
   RSIM_HIP(hipStreamSynchronize(dev->stream), "rsim_sync");
   return RSIM_OK;
 }
 int rsim_timer_begin(rsim_device* dev) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_timer_begin");
   if (rc) return rc;
   RSIM_HIP(hipEventRecord(dev->t0, dev->stream), "hipEventRecord");
   return RSIM_OK;
 }
 int rsim_timer_end(rsim_device* dev, float* ms) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_timer_end");
   if (rc) return rc;
   if (!ms) return fail(RSIM_ERR_INVALID, "rsim_timer_end: null ms");
   RSIM_HIP(hipEventRecord(dev->t1, dev->stream), "hipEventRecord");
@@ -197,7 +212,7 @@ int rsim_timer_end(rsim_device* dev, float* ms) {
 
 // ---- world -------------------------------------------------------------------------------------------------------------------
 int rsim_upload_worlds(rsim_device* dev, const void* static_data, uint32_t count, rsim_world** out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_upload_worlds");
   if (rc) return rc;
   if (!static_data || !out || count == 0) return fail(RSIM_ERR_INVALID, "rsim_upload_worlds: null argument or no world");
   if (count > RSIM_MAX_WORLDS) return fail(RSIM_ERR_INVALID, "rsim_upload_worlds: more than RSIM_MAX_WORLDS worlds");
@@ -212,6 +227,7 @@ int rsim_upload_worlds(rsim_device* dev, const void* static_data, uint32_t count
     delete w;
     return hip_fail(e, "rsim_upload_worlds");
   }
+  ++dev->live;
   *out = w;
   return RSIM_OK;
 }
@@ -222,17 +238,18 @@ uint32_t rsim_world_count(const rsim_world* w) { return w ? w->count : 0; }
 const void* rsim_world_device_ptr(const rsim_world* w) { return w ? w->d : nullptr; }
 int rsim_world_free(rsim_world* w) {
   if (!w) return RSIM_OK;
-  int rc = use(w->dev);
+  int rc = use_outside_capture(w->dev, "rsim_world_free");
   if (rc) return rc;
   (void)hipStreamSynchronize(w->dev->stream);
   (void)hipFree(w->d);
+  --w->dev->live;
   delete w;
   return RSIM_OK;
 }
 
 // ---- states ------------------------------------------------------------------------------------------------------------------
 int rsim_alloc_states(rsim_device* dev, uint32_t n, rsim_states** out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_alloc_states");
   if (rc) return rc;
   if (!out) return fail(RSIM_ERR_INVALID, "rsim_alloc_states: null out");
   size_t bytes = rs::dev::aosoa_words(n > 0 ? n : 1) * sizeof(uint32_t);
@@ -246,11 +263,12 @@ int rsim_alloc_states(rsim_device* dev, uint32_t n, rsim_states** out) {
     delete s;
     return hip_fail(e, "rsim_alloc_states");
   }
+  ++dev->live;
   *out = s;
   return RSIM_OK;
 }
 int rsim_write_states(rsim_device* dev, rsim_states* dst, const void* states) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_write_states");
   if (rc) return rc;
   if (!dst || (!states && dst->n)) return fail(RSIM_ERR_INVALID, "rsim_write_states: null argument");
   std::vector<uint32_t> host;
@@ -284,17 +302,18 @@ int rsim_copy_states(rsim_device* dev, rsim_states* dst, const rsim_states* src)
 }
 int rsim_states_free(rsim_states* s) {
   if (!s) return RSIM_OK;
-  int rc = use(s->dev);
+  int rc = use_outside_capture(s->dev, "rsim_states_free");
   if (rc) return rc;
   (void)hipStreamSynchronize(s->dev->stream);
   (void)hipFree(s->d);
+  --s->dev->live;
   delete s;
   return RSIM_OK;
 }
 uint32_t rsim_states_count(const rsim_states* s) { return s ? s->n : 0; }
 void* rsim_states_device_ptr(rsim_states* s) { return s ? s->d : nullptr; }
 int rsim_download_words(rsim_device* dev, const rsim_states* s, uint32_t* out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_download_words");
   if (rc) return rc;
   if (!s || (!out && s->n)) return fail(RSIM_ERR_INVALID, "rsim_download_words: null argument");
   if (!s->n) return RSIM_OK;
@@ -318,7 +337,7 @@ int rsim_download_states(rsim_device* dev, const rsim_states* s, void* out) {
 
 // ---- word buffers ------------------------------------------------------------------------------------------------------------
 int rsim_alloc_words(rsim_device* dev, uint64_t words, rsim_words** out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_alloc_words");
   if (rc) return rc;
   if (!out) return fail(RSIM_ERR_INVALID, "rsim_alloc_words: null out");
   rsim_words* b = new (std::nothrow) rsim_words{dev, nullptr, words};
@@ -332,11 +351,12 @@ int rsim_alloc_words(rsim_device* dev, uint64_t words, rsim_words** out) {
     delete b;
     return hip_fail(e, "rsim_alloc_words");
   }
+  ++dev->live;
   *out = b;
   return RSIM_OK;
 }
 int rsim_write_words(rsim_device* dev, rsim_words* dst, const uint32_t* host, uint64_t words) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_write_words");
   if (rc) return rc;
   if (!dst || (!host && words)) return fail(RSIM_ERR_INVALID, "rsim_write_words: null argument");
   if (words > dst->n) return fail(RSIM_ERR_SHORT, "rsim_write_words: buffer too short");
@@ -356,7 +376,7 @@ int rsim_upload_words(rsim_device* dev, const uint32_t* host, uint64_t words, rs
   return rc;
 }
 int rsim_read_words(rsim_device* dev, const rsim_words* src, uint32_t* host, uint64_t words) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_read_words");
   if (rc) return rc;
   if (!src || (!host && words)) return fail(RSIM_ERR_INVALID, "rsim_read_words: null argument");
   if (words > src->n) return fail(RSIM_ERR_SHORT, "rsim_read_words: buffer too short");
@@ -372,13 +392,15 @@ int rsim_upload_controls(rsim_device* dev, const void* controls, uint32_t n, rsi
 int rsim_words_free(rsim_words* b) {
   if (!b) return RSIM_OK;
   if (b->view) {
+    --b->dev->live;
     delete b;
     return RSIM_OK;
   }
-  int rc = use(b->dev);
+  int rc = use_outside_capture(b->dev, "rsim_words_free");
   if (rc) return rc;
   (void)hipStreamSynchronize(b->dev->stream);
   (void)hipFree(b->d);
+  --b->dev->live;
   delete b;
   return RSIM_OK;
 }
@@ -387,6 +409,7 @@ int rsim_words_view(rsim_words* parent, uint64_t offset, uint64_t words, rsim_wo
   if (offset > parent->n || words > parent->n - offset) return fail(RSIM_ERR_SHORT, "rsim_words_view: range past the buffer");
   rsim_words* v = new (std::nothrow) rsim_words{parent->dev, parent->d + offset, words, true};
   if (!v) return fail(RSIM_ERR_INVALID, "out of host memory");
+  ++parent->dev->live;
   *out = v;
   return RSIM_OK;
 }
@@ -555,7 +578,14 @@ int rsim_capture_end(rsim_device* dev, rsim_graph** out) {
     (void)hipGraphDestroy(g);
     return hip_fail(e, "hipGraphInstantiate");
   }
-  *out = new rsim_graph{dev, g, x};
+  rsim_graph* gr = new (std::nothrow) rsim_graph{dev, g, x};
+  if (!gr) {
+    (void)hipGraphExecDestroy(x);
+    (void)hipGraphDestroy(g);
+    return fail(RSIM_ERR_INVALID, "out of host memory");
+  }
+  ++dev->live;
+  *out = gr;
   return RSIM_OK;
 }
 int rsim_graph_launch(rsim_device* dev, const rsim_graph* graph) {
@@ -567,15 +597,17 @@ int rsim_graph_launch(rsim_device* dev, const rsim_graph* graph) {
 }
 int rsim_graph_free(rsim_graph* graph) {
   if (!graph) return RSIM_OK;
-  (void)hipSetDevice(graph->dev->ordinal);
+  int rc = use_outside_capture(graph->dev, "rsim_graph_free");
+  if (rc) return rc;
   (void)hipStreamSynchronize(graph->dev->stream);
   (void)hipGraphExecDestroy(graph->exec);
   (void)hipGraphDestroy(graph->graph);
+  --graph->dev->live;
   delete graph;
   return RSIM_OK;
 }
 int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_identity");
   if (rc) return rc;
   if (!in || !out || in->n != out->n) return fail(RSIM_ERR_INVALID, "rsim_identity: buffers of different sizes");
   if (!in->n) return RSIM_OK;
@@ -584,7 +616,7 @@ int rsim_identity(rsim_device* dev, const rsim_states* in, rsim_states* out) {
   return RSIM_OK;
 }
 int rsim_identity_tick(rsim_device* dev, const rsim_states* in, rsim_states* out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_identity_tick");
   if (rc) return rc;
   if (!in || !out || in->n != out->n) return fail(RSIM_ERR_INVALID, "rsim_identity_tick: buffers of different sizes");
   if (!in->n) return RSIM_OK;
@@ -593,7 +625,7 @@ int rsim_identity_tick(rsim_device* dev, const rsim_states* in, rsim_states* out
   return RSIM_OK;
 }
 int rsim_phase(rsim_device* dev, int k, rsim_states* states, const rsim_world* world, int32_t* status_out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_phase");
   if (rc) return rc;
   if (!states || !world) return fail(RSIM_ERR_INVALID, "rsim_phase: null argument");
   if (!states->n) return RSIM_OK;
@@ -602,7 +634,7 @@ int rsim_phase(rsim_device* dev, int k, rsim_states* states, const rsim_world* w
   return finish_status(dev, states->n, status_out);
 }
 int rsim_step_tick(rsim_device* dev, rsim_states* states, const rsim_world* world, int32_t* status_out) {
-  int rc = use(dev);
+  int rc = use_outside_capture(dev, "rsim_step_tick");
   if (rc) return rc;
   if (!states || !world) return fail(RSIM_ERR_INVALID, "rsim_step_tick: null argument");
   if (!states->n) return RSIM_OK;

@@ -11,6 +11,7 @@ present, SimDevice raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -195,8 +196,8 @@ class DeviceStatic:
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
-            lib().rsim_world_free(self.handle)
-            self.handle = None
+            if lib().rsim_world_free(self.handle) == 0:  # refused while the stream records a graph: the buffer stays
+                self.handle = None
 
     def __del__(self):
         try:
@@ -217,8 +218,8 @@ class DeviceStates:
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
-            lib().rsim_states_free(self.handle)
-            self.handle = None
+            if lib().rsim_states_free(self.handle) == 0:  # refused while the stream records a graph: the buffer stays
+                self.handle = None
 
     def __del__(self):
         try:
@@ -238,8 +239,8 @@ class Graph:
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
-            lib().rsim_graph_free(self.handle)
-            self.handle = None
+            if lib().rsim_graph_free(self.handle) == 0:  # refused while the stream records a graph: the buffer stays
+                self.handle = None
 
     def __del__(self):
         try:
@@ -264,12 +265,12 @@ class DeviceWords:
         _check(lib().rsim_words_view(self.handle, offset, n, ctypes.byref(h)), "rsim_words_view")
         v = DeviceWords(self.dev, h.value, n)
         v.parent = self
-        return v
+        return self.dev._own(v)
 
     def free(self):
         if self.handle and self.dev.handle:  # once the device is closed, the memory is released at process exit
-            lib().rsim_words_free(self.handle)
-            self.handle = None
+            if lib().rsim_words_free(self.handle) == 0:  # refused while the stream records a graph: the buffer stays
+                self.handle = None
 
     def __del__(self):
         try:
@@ -279,17 +280,28 @@ class DeviceWords:
 
 
 class SimDevice:
-    """One GPU and its stream (device.rs:30-67)."""
+    """One GPU and its stream (device.rs:30-67). Buffers, worlds and graphs made on it register themselves; `close` frees
+    the ones still open (graphs and views first), as the C ABI's rsim_close requires."""
 
     def __init__(self, ordinal=0):
         h = _vp()
         _check(lib().rsim_open(ordinal, ctypes.byref(h)), "rsim_open")
         self.handle = h
         self.ordinal = ordinal
+        self._owned = weakref.WeakSet()
+
+    def _own(self, obj):
+        self._owned.add(obj)
+        return obj
 
     def close(self):
         if self.handle:
-            lib().rsim_close(self.handle)
+            owned = list(self._owned)
+            # graphs, then views (they only borrow their parent's memory), then buffers and worlds
+            owned.sort(key=lambda o: 0 if isinstance(o, Graph) else 1 if getattr(o, "parent", None) is not None else 2)
+            for o in owned:
+                o.free()
+            _check(lib().rsim_close(self.handle), "rsim_close")
             self.handle = None
 
     def __del__(self):
@@ -306,25 +318,25 @@ class SimDevice:
         h = _vp()
         _check(lib().rsim_upload_world(self.handle, _ptr(_words(world, STATIC_WORDS, "world")), ctypes.byref(h)),
                "rsim_upload_world")
-        return DeviceStatic(self, h)
+        return self._own(DeviceStatic(self, h))
 
     def upload_worlds(self, worlds):
         """A table of StaticData (uint32[count, STATIC_WORDS]) for rsim_sim8_variants."""
         w = _words(worlds).reshape(-1, STATIC_WORDS)
         h = _vp()
         _check(lib().rsim_upload_worlds(self.handle, _ptr(w), w.shape[0], ctypes.byref(h)), "rsim_upload_worlds")
-        return DeviceStatic(self, h)
+        return self._own(DeviceStatic(self, h))
 
     def upload_states(self, states):
         s = _words(states).reshape(-1, ARENA_WORDS)
         h = _vp()
         _check(lib().rsim_upload_states(self.handle, _ptr(s), s.shape[0], ctypes.byref(h)), "rsim_upload_states")
-        return DeviceStates(self, h, s.shape[0])
+        return self._own(DeviceStates(self, h, s.shape[0]))
 
     def alloc_states(self, n):
         h = _vp()
         _check(lib().rsim_alloc_states(self.handle, n, ctypes.byref(h)), "rsim_alloc_states")
-        return DeviceStates(self, h, n)
+        return self._own(DeviceStates(self, h, n))
 
     def write_states(self, d, states):
         s = _words(states, d.n * ARENA_WORDS, "states")
@@ -362,13 +374,13 @@ class SimDevice:
     def alloc_words(self, n):
         h = _vp()
         _check(lib().rsim_alloc_words(self.handle, n, ctypes.byref(h)), "rsim_alloc_words")
-        return DeviceWords(self, h, n)
+        return self._own(DeviceWords(self, h, n))
 
     def upload_words(self, words):
         w = _words(words)
         h = _vp()
         _check(lib().rsim_upload_words(self.handle, _ptr(w), w.size, ctypes.byref(h)), "rsim_upload_words")
-        return DeviceWords(self, h, w.size)
+        return self._own(DeviceWords(self, h, w.size))
 
     def write_words(self, d, words):
         w = _words(words)
@@ -384,7 +396,7 @@ class SimDevice:
         c = _words(controls).reshape(-1, CONTROL_WORDS)
         h = _vp()
         _check(lib().rsim_upload_controls(self.handle, _ptr(c), c.shape[0], ctypes.byref(h)), "rsim_upload_controls")
-        return DeviceWords(self, h, c.size)
+        return self._own(DeviceWords(self, h, c.size))
 
     def sim8(self, states, sd, controls, result, block=SIM8_BLOCK, trace=None):
         """One decision of every arena, in place, asynchronously (device.rs:151-168)."""
@@ -418,7 +430,7 @@ class SimDevice:
         """Stop recording; returns the instantiated Graph."""
         h = _vp()
         _check(lib().rsim_capture_end(self.handle, ctypes.byref(h)), "rsim_capture_end")
-        return Graph(self, h)
+        return self._own(Graph(self, h))
 
     def timer_begin(self):
         _check(lib().rsim_timer_begin(self.handle), "rsim_timer_begin")

@@ -6,6 +6,7 @@ Float policy of both compilers: -ffp-contract=off (no fused multiply-add), no fa
 rounded float divide and sqrt. All HIP code is compiled without packed-fp32 instructions. Everything is compiled with
 -fno-strict-aliasing: the states are copied between buffers and structs through u32 word views.
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -55,16 +56,18 @@ def build(verbose=False):
     """Compiles every source and links librsim.so (always rebuilds)."""
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
+    objs, steps = [], []
     for src in HIP_SOURCES:
         obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([hipcc] + COMMON + HIP_FLAGS + lang + ["-c", src, "-o", obj])
+        steps.append([hipcc] + COMMON + HIP_FLAGS + lang + ["-c", src, "-o", obj])
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-        _run(["g++"] + COMMON + ["-c", src, "-o", obj])
+        steps.append(["g++"] + COMMON + ["-c", src, "-o", obj])
         objs.append(obj)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(len(steps), 8)) as pool:  # the compiles are independent
+        list(pool.map(_run, steps))
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -175,8 +175,8 @@ def order_by_cost(pred_cost, mode, wave=64):
 class Regrouper:
     """Steps decisions of n arenas in a cost-regrouped order, entirely on the device (no host synchronisation, so a
     decision can be captured in a HIP graph). The arenas live in a work buffer in the current order, slot i holding arena
-    where[i]. Each regroup: features -> MLP -> rsim_cost_order -> gather the states into the new order and compose
-    `where`. Each step gathers the controls by `where`, runs sim8 and scatters the result words back to arena order."""
+    where[i]. Each regroup: features -> MLP -> rsim_cost_order -> gather the states into the new order (through a spare
+    buffer, copied back) and compose `where`. Each step gathers the controls by `where`, runs sim8 and scatters the result words back to arena order."""
 
     def __init__(self, dev, model, world, states, mode="sorted", block=SIM8_BLOCK):
         if mode not in ORDER_MODES:
@@ -202,11 +202,15 @@ class Regrouper:
         self.model.predict(self.feat, n, self.pred)
         _check(lib().rsim_cost_order(self.dev.handle, self.pred.handle, n, ORDER_MODES[self.mode], self.work.handle,
                                      self.rel.handle), "rsim_cost_order")
+        # gather into the spare buffers, then copy back: every buffer keeps its role, so a graph that records any number
+        # of regroups replays correctly any number of times (swapping the buffers on the host would bake one direction
+        # into the graph)
         _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, self.rel.handle, self.spare.handle), "rsim_gather_states")
-        self.cur, self.spare = self.spare, self.cur
+        self.dev.copy_states(self.cur, self.spare)
         _check(lib().rsim_permute_words(self.dev.handle, self.where.handle, self.rel.handle, n, 1, 0, self.where2.handle),
                "rsim_permute_words")
-        self.where, self.where2 = self.where2, self.where
+        _check(lib().rsim_permute_words(self.dev.handle, self.where2.handle, self.iota.handle, n, 1, 0, self.where.handle),
+               "rsim_permute_words")
 
     def step(self, controls, result):
         """One decision: controls (DeviceWords, arena order) and result (DeviceWords, arena order)."""

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import rsim.cost as cm
+from conftest import first_diff
 
 # ---- the float64 reference --------------------------------------------------------------------------------------------------
 
@@ -263,7 +264,7 @@ def test_regrouper_in_a_graph(rsim, world, device, dev_world):
         out = [device.alloc_words(n) for _ in range(2)]
         device.sync()
         device.capture_begin()
-        for k in range(2):  # an even number of decisions: the double buffers are back in place after each replay
+        for k in range(2):
             g.regroup()
             g.step(stage[k], out[k])
         graph = g.dev.capture_end()
@@ -356,3 +357,77 @@ def test_words_view(rsim, device):
     assert np.array_equal(device.read_words(base)[:10], np.arange(10))  # freeing a view leaves the memory alone
     with pytest.raises(rsim.RsimError):
         base.view(90, 11)
+
+
+def _regrouped_eager(rsim, device, model, dev_world, states, ctl, mode):
+    g = cm.Regrouper(device, model, dev_world, device.upload_states(states), mode)
+    r = device.alloc_words(states.shape[0])
+    res = []
+    for c in ctl:
+        d_c = device.upload_controls(c)
+        g.regroup()
+        g.step(d_c, r)
+        device.sync()
+        res.append(device.read_words(r))
+    return device.download_states(g.states_in_arena_order()), res
+
+
+@pytest.mark.gpu
+def test_regrouped_decisions_equal_the_oracle(rsim, oracle, world, device, dev_world):
+    """The learned-regrouping workload's mix (1:1 open play and contact) stepped in cost-regrouped order, against the
+    oracle directly: sorted and balanced, eager, and one regroup + decision captured in a graph and replayed three times
+    (an odd count: every buffer keeps its role across replays). Every result word of every decision and every persistent
+    word of the final states."""
+    n, D = 4096, 3
+    note("regrouped decisions vs the oracle, 1:1 open play and contact, %d arenas x %d" % (n, D))
+    states = rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 181, n)
+    ctl = [rsim.sample_controls(182, k, n) for k in range(D)]
+    cpu = states.copy()
+    want_res = [oracle.sim8(cpu, world, c, threads=8) for c in ctl]
+    model = cm.CostModel(device, seed=14)
+    for mode in ("sorted", "balanced"):
+        got, res = _regrouped_eager(rsim, device, model, dev_world, states, ctl, mode)
+        for k in range(D):
+            assert np.array_equal(res[k], want_res[k]), "eager %s decision %d" % (mode, k)
+        fd = first_diff(got[:, :2567], cpu[:, :2567])
+        assert fd is None, "eager %s: arena %d word %d" % ((mode,) + fd)
+        g = cm.Regrouper(device, model, dev_world, device.upload_states(states), mode)
+        stage, out = device.alloc_words(n * rsim.CONTROL_WORDS), device.alloc_words(n)
+        device.sync()
+        device.capture_begin()
+        g.regroup()
+        g.step(stage, out)
+        graph = device.capture_end()
+        for k in range(D):
+            device.write_words(stage, ctl[k])
+            graph.launch()
+            device.sync()
+            assert np.array_equal(device.read_words(out), want_res[k]), "graph %s replay %d" % (mode, k)
+        got = device.download_states(g.states_in_arena_order())
+        fd = first_diff(got[:, :2567], cpu[:, :2567])
+        assert fd is None, "graph %s: arena %d word %d" % ((mode,) + fd)
+        graph.free()
+
+
+@pytest.mark.gpu
+def test_regrouped_full_size(rsim, oracle, world, device, dev_world):
+    """At the learned-regrouping workload's full size (65,536 arenas, 1:1 open play and contact), two decisions regrouped
+    by a trained-shape model in sorted order equal the plain order and the oracle, word for word."""
+    n, D = 65536, 2
+    note("regrouped vs plain vs oracle, %d arenas x %d decisions" % (n, D))
+    states = rsim.sample_arenas(world, rsim.SCEN_OPEN_CONTACT, 191, n)
+    ctl = [rsim.sample_controls(192, k, n) for k in range(D)]
+    model = cm.CostModel(device, seed=15)
+    got, res = _regrouped_eager(rsim, device, model, dev_world, states, ctl, "sorted")
+    plain = device.upload_states(states)
+    r = device.alloc_words(n)
+    for k, c in enumerate(ctl):
+        device.sim8(plain, dev_world, device.upload_controls(c), r)
+        device.sync()
+        assert np.array_equal(device.read_words(r), res[k]), "decision %d" % k
+    assert np.array_equal(device.download_states(plain)[:, :2567], got[:, :2567])
+    cpu = states.copy()
+    for k, c in enumerate(ctl):
+        assert np.array_equal(oracle.sim8(cpu, world, c, threads=16), res[k]), "oracle decision %d" % k
+    fd = first_diff(got[:, :2567], cpu[:, :2567])
+    assert fd is None, "arena %d word %d" % fd

@@ -1,6 +1,6 @@
 """The device API for user kernels (include/rsim_device.h) through its example, rocketsim-cuda_amd/examples/observe.hip:
 its fp16 conversion against numpy's (round to nearest even; NaN -> 0x7FFF by the reference's convention), and on the GPU
-a user kernel that steps and observes against rsim_sim8 and a host computation of the observation."""
+a user kernel that steps and observes against the oracle, rsim_sim8 and a host computation of the observation."""
 import ctypes
 import os
 
@@ -49,11 +49,12 @@ def test_f2h_matches_round_to_nearest_even(obs_lib):
 
 
 @pytest.mark.gpu
-def test_user_kernel_steps_like_sim8_and_observes(rsim, obs_lib, world, device, dev_world):
-    print("[case] example user kernel (step + observe) vs rsim_sim8, 2048 arenas", flush=True)
+def test_user_kernel_steps_like_sim8_and_observes(rsim, oracle, obs_lib, world, device, dev_world):
+    print("[case] example user kernel (step + observe) vs the oracle and rsim_sim8, 2048 arenas", flush=True)
     n = 2048
     states = rsim.sample_arenas(world, rsim.SCEN_MIXED, 181, n)
-    ctl = device.upload_controls(rsim.sample_controls(182, 0, n))
+    host_ctl = rsim.sample_controls(182, 0, n)
+    ctl = device.upload_controls(host_ctl)
     a, b = device.upload_states(states), device.upload_states(states)
     ra, rb = device.alloc_words(n), device.alloc_words(n)
     nobs = obs_lib.observe_words()
@@ -63,6 +64,11 @@ def test_user_kernel_steps_like_sim8_and_observes(rsim, obs_lib, world, device, 
     assert obs_lib.observe_step(device.stream, b.ptr, dev_world.ptr, ctl.ptr, n, rb.ptr, obs.ptr, rew.ptr) == 0
     device.sync()
     sa, sb = device.download_states(a), device.download_states(b)
+    # the user kernel's stepping against the oracle (the CPU build of the tick), then against rsim_sim8
+    cpu = states.copy()
+    cpu_res = oracle.sim8(cpu, world, host_ctl, threads=8)
+    assert np.array_equal(device.read_words(rb), cpu_res)
+    assert np.array_equal(sb[:, :rsim.TICK_WORDS], cpu[:, :rsim.TICK_WORDS])
     assert np.array_equal(device.read_words(ra), device.read_words(rb))
     assert np.array_equal(sa[:, :rsim.TICK_WORDS], sb[:, :rsim.TICK_WORDS])
     got = device.read_words(obs).view(np.uint16)[:n * nobs].reshape(n, nobs)

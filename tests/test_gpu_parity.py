@@ -416,3 +416,49 @@ def test_against_the_reference_build(rsim, world, device, dev_world):
         cs = states.copy()
         cr = [ref.sim8(cs, world, c, threads=8) for c in controls]
         assert_same(gs, gr, cs, cr, "reference build, scenario %d" % scen)
+
+
+def _reference_in(ref, cpu, worlds, variant, controls, threads=8):
+    """One decision of the reference's own build over arenas in per-arena worlds: each world's arenas as one batch."""
+    res = np.zeros(cpu.shape[0], np.uint32)
+    for p in np.unique(variant):
+        idx = np.nonzero(variant == p)[0]
+        sub = np.ascontiguousarray(cpu[idx])
+        res[idx] = ref.sim8(sub, worlds[p], np.ascontiguousarray(controls[idx]), threads=threads)
+        cpu[idx] = sub
+    return res
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                                                    "libref.so")), reason="reference build (oracle/_ref) not available")
+def test_world_table_and_run_against_the_reference_build(rsim, device):
+    """rsim_sim8_variants over a world table of all seven preset pairings with per-world mutators (gravity, boost
+    acceleration, ball radius), and rsim_run (k_run, several decisions in one launch) over the same table, against the
+    reference's own C++ build run world by world: every result word of every decision, every persistent word."""
+    import oracle as orc
+    ref = orc.Reference()
+    rng = np.random.default_rng(6080)
+    worlds = np.stack([rsim.build_world(p, p, float(rng.uniform(-800, -500)), float(rng.uniform(0.8, 1.2)),
+                                        float(rng.uniform(91.25, 100))) for p in range(7) for _ in range(2)])
+    n, D = 4096, 4
+    variant = rng.integers(0, worlds.shape[0], n).astype(np.uint32)
+    states = rsim.sample_arenas_in(worlds, variant, rsim.SCEN_MIXED, 6081)
+    controls = [rsim.sample_controls(6082, d, n) for d in range(D)]
+    dw, dv = device.upload_worlds(worlds), device.upload_words(variant)
+    cs = states.copy()
+    cr = [_reference_in(ref, cs, worlds, variant, c) for c in controls]
+    note("sim8_variants vs the reference build, 14 worlds (7 pairings x mutators), %d arenas x %d decisions" % (n, D))
+    d = device.upload_states(states)
+    r = device.alloc_words(n)
+    gr = []
+    for c in controls:
+        device.sim8_variants(d, dw, dv, device.upload_controls(c), r)
+        device.sync()
+        gr.append(device.read_words(r))
+    assert_same(device.download_states(d), gr, cs, cr, "sim8_variants vs reference build")
+    note("rsim_run (k_run) vs the reference build, the same table, %d decisions in one launch" % D)
+    d = device.upload_states(states)
+    rr = device.alloc_words(D * n)
+    device.run(d, dw, device.upload_words(np.concatenate(controls)), rr, D, variant=dv)
+    device.sync()
+    assert_same(device.download_states(d), list(device.read_words(rr).reshape(D, n)), cs, cr, "k_run vs reference build")
