@@ -166,6 +166,13 @@ int rsim_cost_adam(rsim_device* dev, rsim_words* params, rsim_words* m, rsim_wor
 #define RSIM_ORDER_BALANCED 1
 uint64_t rsim_cost_order_work_words(uint32_t n);
 int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* work, rsim_words* perm);
+/* rsim_cost_order without the work buffer, for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc). No counterpart in the reference. The workspace is a fixed 1 MiB array inside
+ * librsim.so, one per GPU, held until the process ends; n must be <= RSIM_COST_ORDER_NOALLOC_MAX (else RSIM_ERR_INVALID),
+ * and the call allocates nothing. Same results as rsim_cost_order. Every call on one GPU shares that GPU's array: two
+ * orders in flight at once on one GPU (on different streams, or from two rsim_device handles of the same GPU) race. */
+#define RSIM_COST_ORDER_NOALLOC_MAX 262144u
+int rsim_cost_order_noalloc(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* perm);
 /* Uploads a host permutation of 0..n-1 (checked) for the regrouping calls. */
 int rsim_set_permutation(rsim_device* dev, const uint32_t* perm, uint32_t n, rsim_words* dst);
 /* dst arena i = src arena perm[i] for the dst->n arenas of dst (whole AoSoA-64 states; src and dst distinct; perm holds

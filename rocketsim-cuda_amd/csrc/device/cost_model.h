@@ -17,6 +17,11 @@ constexpr uint32_t ORDER_BUCKETS = 256, ORDER_BLOCK = 256, ORDER_SCAN = 1024;
 constexpr float ORDER_HI = 3.0f, ORDER_SCALE = 64.0f;
 constexpr int ORDER_SORTED = 0, ORDER_BALANCED = 1;
 uint64_t order_work_words(uint32_t n);
+// The allocation-free order's fixed workspace: order_work_words(ORDER_SCRATCH_MAX_N) words in a device array of this
+// library, one per GPU; order_scratch_address gives the current device's copy.
+constexpr uint32_t ORDER_SCRATCH_MAX_N = 262144;
+constexpr uint32_t ORDER_SCRATCH_WORDS = ORDER_BUCKETS * ((ORDER_SCRATCH_MAX_N + ORDER_BLOCK - 1) / ORDER_BLOCK);
+hipError_t order_scratch_address(uint32_t** out);
 hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int mode, uint32_t* work, uint32_t* perm);
 
 hipError_t launch_features(hipStream_t st, const uint32_t* states, uint32_t n, float* feat);

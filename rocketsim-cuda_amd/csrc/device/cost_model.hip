@@ -338,6 +338,14 @@ __global__ void __launch_bounds__(ORDER_BLOCK) k_order_place(const float* __rest
 // ---- launchers -----------------------------------------------------------------------------------------------------------------
 static unsigned blocks_of(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
 uint64_t order_work_words(uint32_t n) { return (uint64_t)ORDER_BUCKETS * blocks_of(n, ORDER_BLOCK); }
+// The fixed workspace of launch_cost_order for callers that cannot allocate: the bucket counts of up to ORDER_SCRATCH_MAX_N
+// arenas (1 MiB). Each GPU holds its own copy for the life of the process; host code must not pass this name to a kernel
+// (on the host it is a stand-in, not a device address) but the address order_scratch_address returns for the current
+// device. The counts are overwritten whole by every order (k_order_count writes every (bucket, block) word it scans), so
+// nothing needs clearing between calls; two orders in flight on one GPU at once would race on it.
+static __device__ __align__(256) uint32_t scratch[ORDER_SCRATCH_WORDS];
+hipError_t order_scratch_address(uint32_t** out) { return hipGetSymbolAddress((void**)out, HIP_SYMBOL(scratch)); }
+
 hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int mode, uint32_t* work, uint32_t* perm) {
   const unsigned blocks = blocks_of(n, ORDER_BLOCK);
   k_order_count<<<blocks, ORDER_BLOCK, 0, st>>>(pred, n, work);

@@ -15,6 +15,8 @@
 #include "device/cost_model.h"
 #include "device/launch.h"
 
+static_assert(RSIM_COST_ORDER_NOALLOC_MAX == rs::dev::ORDER_SCRATCH_MAX_N, "the header's limit is the workspace's");
+
 using rs::ArenaState;
 using rs::Controls;
 using rs::StaticData;
@@ -26,6 +28,7 @@ struct rsim_device {
   int32_t* status;  // per-arena status words of the parity kernels
   uint32_t status_cap;
   uint64_t live;  // worlds, state and word buffers (views included) and graphs not yet freed: rsim_close refuses while > 0
+  uint32_t* order_scratch;  // this GPU's copy of the allocation-free order's fixed workspace (rsim_cost_order_noalloc)
 };
 struct rsim_world {
   rsim_device* dev;
@@ -166,6 +169,8 @@ int rsim_open(int ordinal, rsim_device** out) {
   hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&d->t0);
   if (e == hipSuccess) e = hipEventCreate(&d->t1);
+  // the device address of this GPU's fixed order workspace, looked up once with the GPU current
+  if (e == hipSuccess) e = rs::dev::order_scratch_address(&d->order_scratch);
   if (e != hipSuccess) {
     delete d;
     return hip_fail(e, "rsim_open");
@@ -545,6 +550,21 @@ int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mo
   if (!n) return RSIM_OK;
   RSIM_HIP(rs::dev::launch_cost_order(dev->stream, reinterpret_cast<const float*>(pred->d), n, mode, work->d, perm->d),
            "cost order");
+  return RSIM_OK;
+}
+int rsim_cost_order_noalloc(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* perm) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!pred || !perm) return fail(RSIM_ERR_INVALID, "rsim_cost_order_noalloc: null argument");
+  if (mode != RSIM_ORDER_SORTED && mode != RSIM_ORDER_BALANCED)
+    return fail(RSIM_ERR_INVALID, "rsim_cost_order_noalloc: unknown mode");
+  if (n > RSIM_COST_ORDER_NOALLOC_MAX)
+    return fail(RSIM_ERR_INVALID, "rsim_cost_order_noalloc: n is above RSIM_COST_ORDER_NOALLOC_MAX (262,144 arenas)");
+  if (pred->n < 2ull * n || perm->n < n) return fail(RSIM_ERR_SHORT, "rsim_cost_order_noalloc: buffers too short");
+  if (!n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_cost_order(dev->stream, reinterpret_cast<const float*>(pred->d), n, mode, dev->order_scratch,
+                                      perm->d),
+           "cost order (fixed workspace)");
   return RSIM_OK;
 }
 int rsim_permute_words(rsim_device* dev, const rsim_words* src, const rsim_words* perm, uint32_t n, uint32_t words_per_item,

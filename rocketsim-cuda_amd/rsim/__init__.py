@@ -72,6 +72,7 @@ def lib():
             "rsim_set_permutation": ([_vp, _vp, u32, _vp], ctypes.c_int),
             "rsim_cost_order_work_words": ([u32], u64),
             "rsim_cost_order": ([_vp, _vp, u32, ctypes.c_int, _vp, _vp], ctypes.c_int),
+            "rsim_cost_order_noalloc": ([_vp, _vp, u32, ctypes.c_int, _vp], ctypes.c_int),
             "rsim_capture_begin": ([_vp], ctypes.c_int), "rsim_capture_end": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
             "rsim_graph_launch": ([_vp, _vp], ctypes.c_int), "rsim_graph_free": ([_vp], ctypes.c_int),
             "rsim_gather_states": ([_vp, _vp, _vp, _vp], ctypes.c_int),
