@@ -333,9 +333,12 @@ RS_FN double atan2_64(double y, double x) {
   if (!swap && !xneg && num / den < 0x1p-300) return y / x;  // atan(z) = z to far below an ulp
   int e = (int)(dbits(den) >> 52) - 1023;  // bring the pair to the unit range (exact power-of-two scaling)
   if (e > 400 || e < -400) {
-    double s = dfrom((uint64_t)(1023 - e) << 52);
-    num *= s;
-    den *= s;
+    // 2^-e as two factors, each a normal binary64 for every e of a finite den (2^-1023 alone is not); den lands in [1, 2)
+    // exactly, num too unless it falls far below an ulp of the result (then it is replaced just below)
+    const int e1 = e / 2, e2 = e - e1;
+    const double s1 = dfrom((uint64_t)(1023 - e1) << 52), s2 = dfrom((uint64_t)(1023 - e2) << 52);
+    num = num * s1 * s2;
+    den = den * s1 * s2;
     if (num < 0x1p-600) num = 0x1p-600 * (num > 0.0 ? 1.0 : 0.0);  // far below an ulp of any result here
   }
   DD v = atan_dd(quot_dd(num, den));
