@@ -144,3 +144,53 @@ def test_attribution_with_the_platform_build_as_target(oracle, mixed, world, ope
     planted = Attribution()
     attribute_step(mixed, planted, "planted", s, world, dev, target, _tick(oracle, ctl[0]))
     assert (planted.words, planted.other, sum(planted.by_func), planted.combo) == (1, 1, 0, 0)
+
+
+def _phase_step(k):
+    def step(m, s, w, mask):
+        m.phase(k, s, w, mask)
+    return step
+
+
+@pytest.mark.gpu
+def test_device_attribution_against_the_platform_build(rsim, oracle, mixed, world, device, dev_world):
+    """The attribution's device leg (crates/rocketsim-cuda/tests/common/mod.rs:458-541): the GPU's phases against the
+    platform build (the stand-in for V3, which computes with this host's C library), phase by phase from the same input,
+    following the platform build's states as the reference's phase driver follows V3's. Every word in which the GPU differs
+    is charged to a libm function (or two together), none to `other`; a planted non-libm difference is."""
+    print("[case] device phases vs the platform build, 192 arenas x 3 ticks x 10 phases", flush=True)
+    st = np.concatenate([rsim.sample_arenas(world, rsim.SCEN_OPEN, 21, 128), rsim.sample_arenas(world, rsim.SCEN_CONTACT, 22, 64)])
+    ctl = np.concatenate([rsim.sample_controls(21, 0, 128), rsim.sample_controls(22, 0, 64)])
+    n = st.shape[0]
+    acc = Attribution()
+    diffs = 0
+    cur = st.copy()
+    for t in range(3):
+        for a in range(n):
+            oracle.set_controls(cur[a], ctl[a])
+        for k in range(10):
+            d = device.upload_states(cur)
+            device.phase(k, d, dev_world)
+            dev = device.download_states(d)
+            target = cur.copy()
+            for a in range(n):
+                mixed.phase(k, target[a], world, orc.PLATFORM_ALL)
+            for a in range(n):
+                attribute_step(mixed, acc, "arena %d tick %d phase %d" % (a, t, k), cur[a], world, dev[a], target[a],
+                               _phase_step(k))
+                diffs += int(not np.array_equal(dev[a], target[a]))
+            cur = target
+    print("[attribution] " + str(acc.row()), flush=True)
+    assert acc.ran == n * 3 * 10 and acc.states_diff == diffs > 0
+    assert acc.other == 0, acc.examples
+    assert acc.words > 0 and sum(acc.by_func) + acc.combo >= acc.words
+    # a planted difference in the device's output is not the libm's
+    d = device.upload_states(st[:1])
+    device.phase(0, d, dev_world)
+    dev = device.download_states(d)[0]
+    target = st[0].copy()
+    mixed.phase(0, target, world, orc.PLATFORM_ALL)
+    dev[100] ^= 1
+    planted = Attribution()
+    attribute_step(mixed, planted, "planted", st[0], world, dev, target, _phase_step(0))
+    assert planted.other >= 1
