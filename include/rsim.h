@@ -140,6 +140,24 @@ int rsim_sim8_timed(rsim_device* dev, rsim_states* states, const rsim_world* wor
 int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant, const rsim_words* controls,
              rsim_words* results, uint32_t decisions, uint32_t block);
 
+/* rsim_run with dynamic scheduling: `workgroups` persistent work-groups of `block` lanes (64, 128, 192 or 256: whole
+ * waves; workgroups 0 = as many as the GPU holds at once, rsim_run_queue_capacity) whose waves each take the next AoSoA
+ * block of 64 arenas from a queue and run all `decisions` decisions of it. The queue holds the ceil(n / 64) blocks in
+ * the order `order` (a permutation of the block indices, e.g. rsim_cost_block_order's heaviest-first order) or in index
+ * order (order null). Same results and states as rsim_run: arenas are independent and each is stepped by one lane.
+ * Asynchronous; calls on one device are ordered by its stream (the queue counter is the device's). No counterpart in
+ * the reference. */
+int rsim_run_queued(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* results, uint32_t decisions, uint32_t block, const rsim_words* order,
+                    uint32_t workgroups);
+uint32_t rsim_run_queue_capacity(rsim_device* dev, uint32_t block); /* resident work-groups of that size (0: unknown) */
+/* One decision as ceil(8 / ticks_per_launch) launches of ticks_per_launch ticks, the state written back to the AoSoA
+ * buffer between launches (ticks_per_launch 8 is one launch, as rsim_sim8_variants). Same result words and states as
+ * rsim_sim8_variants; a schedule the autotuner measures. Asynchronous. No counterpart in the reference (its sim8 is one
+ * launch per decision; step_tick is its one-tick debug kernel, device.rs:96-141). */
+int rsim_sim8_split(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* result, uint32_t block, uint32_t ticks_per_launch);
+
 /* ---- the learned cost model and arena regrouping (no counterpart in the reference) ----------------------------------------
  * A 48 -> 64 -> 64 -> 2 MLP over per-arena features predicts (log of the arena's decision cost, contact logit). Inference
  * runs on bf16 MFMA with fp32 accumulation; training is the library's own forward/backward/Adam kernels. Parameters: a
@@ -166,6 +184,10 @@ int rsim_cost_adam(rsim_device* dev, rsim_words* params, rsim_words* m, rsim_wor
 #define RSIM_ORDER_BALANCED 1
 uint64_t rsim_cost_order_work_words(uint32_t n);
 int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* work, rsim_words* perm);
+/* The order of the ceil(n / 64) AoSoA blocks by their largest predicted arena cost (pred as for rsim_cost_order), most
+ * expensive first, stable: the queue order of rsim_run_queued (longest work first). work: rsim_cost_block_order_work_words(n). */
+uint64_t rsim_cost_block_order_work_words(uint32_t n);
+int rsim_cost_block_order(rsim_device* dev, const rsim_words* pred, uint32_t n, rsim_words* work, rsim_words* order);
 /* rsim_cost_order without the work buffer, for callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops that forbid hipMalloc). No counterpart in the reference. The workspace is a fixed 1 MiB array inside
  * librsim.so, one per GPU, held until the process ends; n must be <= RSIM_COST_ORDER_NOALLOC_MAX (else RSIM_ERR_INVALID),

@@ -24,6 +24,10 @@ constexpr uint32_t ORDER_SCRATCH_WORDS = ORDER_BUCKETS * ((ORDER_SCRATCH_MAX_N +
 hipError_t order_scratch_address(uint32_t** out);
 hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int mode, uint32_t* work, uint32_t* perm);
 
+// The order of the ceil(n / 64) AoSoA blocks by their largest predicted arena cost, most expensive first (stable):
+// work holds block_order_work_words(n) words.
+uint64_t block_order_work_words(uint32_t n);
+hipError_t launch_block_order(hipStream_t st, const float* pred, uint32_t n, uint32_t* work, uint32_t* order);
 hipError_t launch_features(hipStream_t st, const uint32_t* states, uint32_t n, float* feat);
 hipError_t launch_mlp_infer(hipStream_t st, const float* params, const float* feat, uint32_t n, float* out);
 hipError_t launch_mlp_train(hipStream_t st, const float* params, const float* feat, const float* target, const float* contact,

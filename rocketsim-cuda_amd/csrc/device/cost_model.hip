@@ -335,6 +335,21 @@ __global__ void __launch_bounds__(ORDER_BLOCK) k_order_place(const float* __rest
   if (s < n) perm[order_slot(s, n, mode)] = a;  // s < n always (the counts sum to n); the check keeps a stray rank out
 }
 
+// The cost of an AoSoA block of 64 arenas for the block queue (k_run_queue): the largest predicted cost among its arenas
+// (a wave lasts about as long as its slowest lane), written as a prediction row (cost, 0) so the block order is the
+// arenas' counting sort applied to blocks. NaN predictions are skipped (a block of NaNs keeps NaN, which sorts last).
+__global__ void __launch_bounds__(256) k_block_cost(const float* __restrict__ pred, uint32_t n, float* __restrict__ out) {
+  const uint32_t b = blockIdx.x * 256 + threadIdx.x, nb = (n + AOSOA_LANES - 1) / AOSOA_LANES;
+  if (b >= nb) return;
+  float m = __builtin_nanf("");
+  for (uint32_t a = b * AOSOA_LANES; a < n && a < (b + 1) * AOSOA_LANES; ++a) {
+    const float c = pred[2 * (size_t)a];
+    if (c == c && !(c <= m)) m = c;  // m NaN: the first non-NaN cost; afterwards the larger
+  }
+  out[2 * (size_t)b] = m;
+  out[2 * (size_t)b + 1] = 0.0f;
+}
+
 // ---- launchers -----------------------------------------------------------------------------------------------------------------
 static unsigned blocks_of(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
 uint64_t order_work_words(uint32_t n) { return (uint64_t)ORDER_BUCKETS * blocks_of(n, ORDER_BLOCK); }
@@ -352,6 +367,16 @@ hipError_t launch_cost_order(hipStream_t st, const float* pred, uint32_t n, int 
   k_order_scan<<<1, ORDER_SCAN, 0, st>>>(work, (uint32_t)order_work_words(n));
   k_order_place<<<blocks, ORDER_BLOCK, 0, st>>>(pred, n, work, mode, perm);
   return hipGetLastError();
+}
+uint64_t block_order_work_words(uint32_t n) {
+  const uint32_t nb = (n + AOSOA_LANES - 1) / AOSOA_LANES;
+  return 2ull * nb + order_work_words(nb);
+}
+hipError_t launch_block_order(hipStream_t st, const float* pred, uint32_t n, uint32_t* work, uint32_t* order) {
+  const uint32_t nb = (n + AOSOA_LANES - 1) / AOSOA_LANES;
+  float* block_pred = reinterpret_cast<float*>(work);
+  k_block_cost<<<blocks_of(nb, 256), 256, 0, st>>>(pred, n, block_pred);
+  return launch_cost_order(st, block_pred, nb, ORDER_SORTED, work + 2ull * nb, order);
 }
 hipError_t launch_features(hipStream_t st, const uint32_t* states, uint32_t n, float* feat) {
   k_features<<<blocks_of(n, 256), 256, 0, st>>>(states, n, feat);

@@ -36,6 +36,14 @@ struct Sim8Args {
 };
 hipError_t launch_sim8(hipStream_t st, const Sim8Args& p, uint32_t block);
 hipError_t launch_run(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block);  // ctl/result: per decision
+// One decision as ceil(8 / ticks_per_launch) launches of k_sim8_ticks (no trace, no cycles).
+hipError_t launch_sim8_ticks(hipStream_t st, const Sim8Args& p, uint32_t block, uint32_t ticks_per_launch);
+// k_run_queue: `workgroups` persistent work-groups (block a multiple of 64), the queue counter `next` reset on the stream
+// first; order: null or a permutation of the ceil(n / 64) block indices.
+hipError_t launch_run_queue(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block, const uint32_t* order,
+                            uint32_t* next, uint32_t workgroups);
+// How many work-groups of `block` lanes of k_run_queue are resident at once on the current GPU (0 if unknown).
+uint32_t run_queue_capacity(uint32_t block);
 
 }  // namespace dev
 }  // namespace rs

@@ -29,6 +29,7 @@ struct rsim_device {
   uint32_t status_cap;
   uint64_t live;  // worlds, state and word buffers (views included) and graphs not yet freed: rsim_close refuses while > 0
   uint32_t* order_scratch;  // this GPU's copy of the allocation-free order's fixed workspace (rsim_cost_order_noalloc)
+  uint32_t* queue;          // the block queue's counter of rsim_run_queued (one word, reset on the stream by each launch)
 };
 struct rsim_world {
   rsim_device* dev;
@@ -171,7 +172,9 @@ int rsim_open(int ordinal, rsim_device** out) {
   if (e == hipSuccess) e = hipEventCreate(&d->t1);
   // the device address of this GPU's fixed order workspace, looked up once with the GPU current
   if (e == hipSuccess) e = rs::dev::order_scratch_address(&d->order_scratch);
+  if (e == hipSuccess) e = hipMalloc((void**)&d->queue, 256);
   if (e != hipSuccess) {
+    if (d->queue) (void)hipFree(d->queue);
     delete d;
     return hip_fail(e, "rsim_open");
   }
@@ -186,6 +189,7 @@ int rsim_close(rsim_device* dev) {
     return fail(RSIM_ERR_INVALID, "rsim_close: worlds, buffers or graphs of this device are still open (free them first)");
   (void)hipStreamSynchronize(dev->stream);
   if (dev->status) (void)hipFree(dev->status);
+  if (dev->queue) (void)hipFree(dev->queue);
   (void)hipEventDestroy(dev->t0);
   (void)hipEventDestroy(dev->t1);
   (void)hipStreamDestroy(dev->stream);
@@ -465,6 +469,51 @@ int rsim_run(rsim_device* dev, rsim_states* states, const rsim_world* world, con
   RSIM_HIP(rs::dev::launch_run(dev->stream, p, decisions, block), "run launch");
   return RSIM_OK;
 }
+int rsim_run_queued(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* results, uint32_t decisions, uint32_t block, const rsim_words* order,
+                    uint32_t workgroups) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world || !controls || !results) return fail(RSIM_ERR_INVALID, "rsim_run_queued: null argument");
+  if (block < 64 || block > RSIM_MAX_BLOCK || block % 64)
+    return fail(RSIM_ERR_INVALID, "rsim_run_queued: block must be 64, 128, 192 or 256 (whole waves)");
+  const uint32_t n = states->n, nblocks = (n + 63) / 64;
+  if (controls->n < (uint64_t)decisions * n * rs::NUM_CARS * 8 || results->n < (uint64_t)decisions * n)
+    return fail(RSIM_ERR_SHORT, "rsim_run_queued: buffers too short");
+  if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_run_queued: variant buffer too short");
+  if (order && order->n < nblocks) return fail(RSIM_ERR_SHORT, "rsim_run_queued: order shorter than the block count");
+  if (!n || !decisions) return RSIM_OK;
+  if (!workgroups) workgroups = rs::dev::run_queue_capacity(block);
+  const uint32_t waves = block / 64;
+  const uint32_t useful = (nblocks + waves - 1) / waves;  // more work-groups than that would find the queue empty
+  if (!workgroups || workgroups > useful) workgroups = useful;
+  rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
+                      reinterpret_cast<const Controls*>(controls->d), results->d, n, nullptr, nullptr};
+  RSIM_HIP(rs::dev::launch_run_queue(dev->stream, p, decisions, block, order ? order->d : nullptr, dev->queue, workgroups),
+           "run_queued launch");
+  return RSIM_OK;
+}
+int rsim_sim8_split(rsim_device* dev, rsim_states* states, const rsim_world* world, const rsim_words* variant,
+                    const rsim_words* controls, rsim_words* result, uint32_t block, uint32_t ticks_per_launch) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!states || !world || !controls || !result) return fail(RSIM_ERR_INVALID, "rsim_sim8_split: null argument");
+  if (block < 1 || block > RSIM_MAX_BLOCK) return fail(RSIM_ERR_INVALID, "rsim_sim8_split: block outside 1..RSIM_MAX_BLOCK");
+  if (ticks_per_launch < 1 || ticks_per_launch > RSIM_TICKS_PER_DECISION)
+    return fail(RSIM_ERR_INVALID, "rsim_sim8_split: ticks_per_launch outside 1..8");
+  const uint32_t n = states->n;
+  if (controls->n < (uint64_t)n * rs::NUM_CARS * 8 || result->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8_split: buffers too short");
+  if (variant && variant->n < n) return fail(RSIM_ERR_SHORT, "rsim_sim8_split: variant buffer too short");
+  if (!n) return RSIM_OK;
+  rs::dev::Sim8Args p{states->d, world->d, world->count, variant ? variant->d : nullptr,
+                      reinterpret_cast<const Controls*>(controls->d), result->d, n, nullptr, nullptr};
+  RSIM_HIP(rs::dev::launch_sim8_ticks(dev->stream, p, block, ticks_per_launch), "sim8_split launch");
+  return RSIM_OK;
+}
+uint32_t rsim_run_queue_capacity(rsim_device* dev, uint32_t block) {
+  if (use(dev) || block < 64 || block > RSIM_MAX_BLOCK || block % 64) return 0;
+  return rs::dev::run_queue_capacity(block);
+}
 // ---- the cost model and regrouping -------------------------------------------------------------------------------------------
 int rsim_cost_features(rsim_device* dev, const rsim_states* states, rsim_words* features) {
   int rc = use(dev);
@@ -550,6 +599,17 @@ int rsim_cost_order(rsim_device* dev, const rsim_words* pred, uint32_t n, int mo
   if (!n) return RSIM_OK;
   RSIM_HIP(rs::dev::launch_cost_order(dev->stream, reinterpret_cast<const float*>(pred->d), n, mode, work->d, perm->d),
            "cost order");
+  return RSIM_OK;
+}
+uint64_t rsim_cost_block_order_work_words(uint32_t n) { return rs::dev::block_order_work_words(n); }
+int rsim_cost_block_order(rsim_device* dev, const rsim_words* pred, uint32_t n, rsim_words* work, rsim_words* order) {
+  int rc = use(dev);
+  if (rc) return rc;
+  if (!pred || !work || !order) return fail(RSIM_ERR_INVALID, "rsim_cost_block_order: null argument");
+  if (pred->n < 2ull * n || order->n < (n + 63) / 64 || work->n < rs::dev::block_order_work_words(n))
+    return fail(RSIM_ERR_SHORT, "rsim_cost_block_order: buffers too short");
+  if (!n) return RSIM_OK;
+  RSIM_HIP(rs::dev::launch_block_order(dev->stream, reinterpret_cast<const float*>(pred->d), n, work->d, order->d), "block order");
   return RSIM_OK;
 }
 int rsim_cost_order_noalloc(rsim_device* dev, const rsim_words* pred, uint32_t n, int mode, rsim_words* perm) {

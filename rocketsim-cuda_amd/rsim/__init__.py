@@ -64,6 +64,11 @@ def lib():
             "rsim_upload_worlds": ([_vp, _vp, u32, ctypes.POINTER(_vp)], ctypes.c_int), "rsim_world_count": ([_vp], u32),
             "rsim_sim8_variants": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, _vp], ctypes.c_int),
             "rsim_run": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, u32], ctypes.c_int),
+            "rsim_run_queued": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, u32, _vp, u32], ctypes.c_int),
+            "rsim_run_queue_capacity": ([_vp, u32], u32),
+            "rsim_sim8_split": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, u32], ctypes.c_int),
+            "rsim_cost_block_order_work_words": ([u32], u64),
+            "rsim_cost_block_order": ([_vp, _vp, u32, _vp, _vp], ctypes.c_int),
             "rsim_sim8_timed": ([_vp, _vp, _vp, _vp, _vp, _vp, u32, _vp, _vp], ctypes.c_int),
             "rsim_cost_features": ([_vp, _vp, _vp], ctypes.c_int),
             "rsim_cost_predict": ([_vp, _vp, _vp, u32, _vp], ctypes.c_int),
@@ -414,6 +419,22 @@ class SimDevice:
         """`decisions` decisions in one launch (controls: decisions x n x 16 words, results: decisions x n words)."""
         _check(lib().rsim_run(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
                               controls.handle, results.handle, decisions, block), "rsim_run")
+
+    def run_queued(self, states, worlds, controls, results, decisions, variant=None, block=SIM8_BLOCK, order=None,
+                   workgroups=0):
+        """rsim_run with persistent work-groups taking AoSoA blocks from a queue (in `order`, a DeviceWords permutation of
+        the block indices, or in index order); workgroups 0: as many as the GPU holds."""
+        _check(lib().rsim_run_queued(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
+                                     controls.handle, results.handle, decisions, block,
+                                     order.handle if order is not None else None, workgroups), "rsim_run_queued")
+
+    def run_queue_capacity(self, block=SIM8_BLOCK):
+        return int(lib().rsim_run_queue_capacity(self.handle, block))
+
+    def sim8_split(self, states, worlds, controls, result, ticks_per_launch, variant=None, block=SIM8_BLOCK):
+        """One decision as launches of `ticks_per_launch` ticks (the state back in HBM between launches)."""
+        _check(lib().rsim_sim8_split(self.handle, states.handle, worlds.handle, variant.handle if variant is not None else None,
+                                     controls.handle, result.handle, block, ticks_per_launch), "rsim_sim8_split")
 
     def gather_states(self, src, perm, dst):
         """dst arena i = src arena perm[i] for the dst.n arenas of dst (asynchronous). perm is a DeviceWords; the caller

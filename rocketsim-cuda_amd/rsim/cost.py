@@ -178,7 +178,9 @@ class Regrouper:
     where[i]. Each regroup: features -> MLP -> rsim_cost_order -> gather the states into the new order (through a spare
     buffer, copied back) and compose `where`. Each step gathers the controls by `where`, runs sim8 and scatters the result words back to arena order."""
 
-    def __init__(self, dev, model, world, states, mode="sorted", block=SIM8_BLOCK):
+    def __init__(self, dev, model, world, states, mode="sorted", block=SIM8_BLOCK, variant=None, max_decisions=1):
+        """world: a one-world buffer, or a world table with `variant` (DeviceWords, arena order) naming each arena's
+        world; max_decisions: the most decisions one `run` call takes (its regrouped controls and results are held)."""
         if mode not in ORDER_MODES:
             raise ValueError(mode)
         self.dev, self.model, self.world, self.mode, self.block = dev, model, world, mode, block
@@ -195,6 +197,28 @@ class Regrouper:
         self.pred = dev.alloc_words(2 * n)
         self.ctl = dev.alloc_words(n * CONTROL_WORDS)
         self.res = dev.alloc_words(n)
+        self.variant = variant  # arena order
+        self.var_cur = dev.alloc_words(n) if variant is not None else None  # slot order
+        if variant is not None:
+            self._permute(variant, self.where, 1, False, self.var_cur)
+        self.max_decisions = max_decisions
+        if max_decisions > 1:
+            self.ctl_run = dev.alloc_words(max_decisions * n * CONTROL_WORDS)
+            self.res_run = dev.alloc_words(max_decisions * n)
+            self.ctl_run_v = [self.ctl_run.view(k * n * CONTROL_WORDS, n * CONTROL_WORDS) for k in range(max_decisions)]
+            self.res_run_v = [self.res_run.view(k * n, n) for k in range(max_decisions)]
+        self._views = {}  # per-decision views of the caller's buffers, made once and kept (views are not made in a capture)
+
+    def _permute(self, src, perm, words, scatter, dst):
+        _check(lib().rsim_permute_words(self.dev.handle, src.handle, perm.handle, self.n, words, 1 if scatter else 0, dst.handle),
+               "rsim_permute_words")
+    def reset(self, states):
+        """Back to arena order with the states of `states` (a buffer in arena order): copies them into the work buffer
+        and resets `where` (asynchronous and capturable, so a recorded sequence can start with it and be replayed)."""
+        self.dev.copy_states(self.cur, states)
+        self._permute(self.iota, self.iota, 1, False, self.where)
+        if self.variant is not None:
+            self._permute(self.variant, self.iota, 1, False, self.var_cur)
 
     def regroup(self):
         n = self.n
@@ -211,15 +235,61 @@ class Regrouper:
                "rsim_permute_words")
         _check(lib().rsim_permute_words(self.dev.handle, self.where2.handle, self.iota.handle, n, 1, 0, self.where.handle),
                "rsim_permute_words")
+        if self.variant is not None:
+            self._permute(self.variant, self.where, 1, False, self.var_cur)
 
     def step(self, controls, result):
         """One decision: controls (DeviceWords, arena order) and result (DeviceWords, arena order)."""
         n = self.n
         _check(lib().rsim_permute_words(self.dev.handle, controls.handle, self.where.handle, n, CONTROL_WORDS, 0, self.ctl.handle),
                "rsim_permute_words")
-        self.dev.sim8(self.cur, self.world, self.ctl, self.res, self.block)
+        self.dev.sim8_variants(self.cur, self.world, self.var_cur, self.ctl, self.res, self.block)
         _check(lib().rsim_permute_words(self.dev.handle, self.res.handle, self.where.handle, n, 1, 1, result.handle),
                "rsim_permute_words")
+
+    def views(self, controls, results, decisions):
+        """Per-decision views of a decisions x n x 16 controls buffer and a decisions x n results buffer (made once per
+        buffer and kept: call this before a capture that records `run` on them)."""
+        key = (id(controls), id(results), decisions)
+        if key not in self._views:
+            n = self.n
+            self._views[key] = ([controls.view(k * n * CONTROL_WORDS, n * CONTROL_WORDS) for k in range(decisions)],
+                                [results.view(k * n, n) for k in range(decisions)], controls, results)
+        return self._views[key][:2]
+
+    def run(self, controls, results, decisions, queued=True):
+        """`decisions` (<= max_decisions) decisions in one launch in the current order: controls decisions x n x 16 words
+        and results decisions x n words, both in arena order. The launch is rsim_run_queued, whose queue takes the blocks
+        in index order (after a sorted regroup: heaviest first), or rsim_run."""
+        if decisions > self.max_decisions:
+            raise ValueError("run: %d decisions, the regrouper holds %d" % (decisions, self.max_decisions))
+        cv, rv = self.views(controls, results, decisions)
+        for k in range(decisions):
+            self._permute(cv[k], self.where, CONTROL_WORDS, False, self.ctl_run_v[k])
+        ctl, res = (self.ctl_run, self.res_run) if decisions == self.max_decisions else self._run_views(decisions)
+        if queued:
+            self.dev.run_queued(self.cur, self.world, ctl, res, decisions, variant=self.var_cur, block=self.block)
+        else:
+            self.dev.run(self.cur, self.world, ctl, res, decisions, variant=self.var_cur, block=self.block)
+        for k in range(decisions):
+            self._permute(self.res_run_v[k], self.where, 1, True, rv[k])
+
+    def _run_views(self, decisions):
+        key = ("run", decisions)
+        if key not in self._views:
+            n = self.n
+            self._views[key] = (self.ctl_run.view(0, decisions * n * CONTROL_WORDS), self.res_run.view(0, decisions * n))
+        return self._views[key]
+
+    def slots_of(self, arenas, out):
+        """out[i] = the slot that holds arena arenas[i] now, for the n entries of `arenas` (DeviceWords of n arena indices,
+        e.g. a sample padded with zeros) into `out` (n words); asynchronous and capturable. Gathering the first m slots
+        with rsim_gather_states then reads a sample of m arenas from the regrouped buffer. Call once before a capture
+        (it allocates its inverse map on the first call)."""
+        if not hasattr(self, "_inv"):
+            self._inv = self.dev.alloc_words(self.n)
+        self._permute(self.iota, self.where, 1, True, self._inv)  # inv[where[i]] = i
+        self._permute(self._inv, arenas, 1, False, out)
 
     def states_in_arena_order(self, out=None):
         """A states buffer with arena a at position a (asynchronous)."""
@@ -230,3 +300,23 @@ class Regrouper:
         _check(lib().rsim_gather_states(self.dev.handle, self.cur.handle, inv.handle, out.handle), "rsim_gather_states")
         self.dev.sync()
         return out
+
+
+class BlockOrder:
+    """The heaviest-first order of the AoSoA blocks of a state buffer by the cost model (features -> MLP ->
+    rsim_cost_block_order), for rsim_run_queued's queue: scheduling by predicted cost without moving any arena.
+    Asynchronous and capturable; the buffers are allocated here, before any capture."""
+
+    def __init__(self, dev, model, states):
+        self.dev, self.model, self.states, self.n = dev, model, states, states.n
+        self.feat = dev.alloc_words(self.n * FEATURES)
+        self.pred = dev.alloc_words(2 * self.n)
+        self.work = dev.alloc_words(int(lib().rsim_cost_block_order_work_words(self.n)))
+        self.order = dev.alloc_words((self.n + 63) // 64)
+
+    def update(self):
+        self.model.features(self.states, self.feat)
+        self.model.predict(self.feat, self.n, self.pred)
+        _check(lib().rsim_cost_block_order(self.dev.handle, self.pred.handle, self.n, self.work.handle, self.order.handle),
+               "rsim_cost_block_order")
+        return self.order
