@@ -18,8 +18,14 @@ struct Np {
   TravBatch* tb;
 };
 RS_FN Aff body_trans(const Np& c, uint32_t b) {
-  int d = dyn_slot(c.w, b);
-  return d >= 0 ? c.s.bodies[d].world : c.w.bodies[b].pose;
+  // Both candidates are loaded, each from its own memory (a dynamic body's transform from the arena state, a static
+  // body's pose from the world), and one is picked. A load in each branch would be merged by the compiler into one load
+  // through a conditional address, mixing the two memories into a generic pointer whose loads are flat instructions.
+  // d < 0 reads slot 0, which exists, and discards it.
+  const int d = dyn_slot(c.w, b);
+  const Aff dyn = c.s.bodies[d >= 0 ? d : 0].world;
+  const Aff fixed = c.w.bodies[b].pose;
+  return d >= 0 ? dyn : fixed;
 }
 RS_FN uint32_t body_flags(const Np& c, uint32_t b) {
   int d = dyn_slot(c.w, b);

@@ -786,7 +786,7 @@ RS_FN int mesh_job_of(const Np& c, uint32_t a, uint32_t b, uint32_t* body, uint3
   return mesh_of(c.w, other) ? TJ_CAR : -1;
 }
 // P3b (:230-282): the retained pairs in pair-array order. Returns the number of unimplemented pair algorithms reached.
-RS_FN uint32_t phase_p3b(ArenaState& s, const StaticData& w) {
+RS_INLINE uint32_t phase_p3b_body(ArenaState& s, const StaticData& w) {
   Np c = {s, w, 0, nullptr, nullptr};
   uint32_t n = s.n_pairs < (uint32_t)MAX_PAIRS ? s.n_pairs : (uint32_t)MAX_PAIRS;
   TravBatch tb;
@@ -815,6 +815,15 @@ RS_FN uint32_t phase_p3b(ArenaState& s, const StaticData& w) {
     near_callback(c, w.proxies[p.proxy0].client_body, w.proxies[p.proxy1].client_body);
   }
   return c.unimplemented;
+}
+// The phase as a function of its own whose state and world pointers carry their address spaces: its accesses compile to
+// scratch_ and global_ instructions (through generic pointers they would be flat ones, which wait on both the vector-memory
+// and the LDS counters and need 64-bit addresses).
+RS_NOINL uint32_t phase_p3b_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp) {
+  return phase_p3b_body(*(ArenaState*)sp, *(const StaticData*)wp);
+}
+RS_FN uint32_t phase_p3b(ArenaState& s, const StaticData& w) {
+  return phase_p3b_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w);
 }
 
 }  // namespace rs

@@ -31,15 +31,16 @@ RS_FN BodyView body_view(const ArenaState& s, const StaticData& w, uint32_t body
   int d = dyn_slot(w, body);
   v.user_idx = c.user_idx;
   v.inv_mass = c.inv_mass;
+  // both transforms loaded and one picked (see body_trans: no load through a conditional address)
+  const Aff dyn_world = s.bodies[d >= 0 ? d : 0].world, pose = c.pose;
+  v.world = d >= 0 ? dyn_world : pose;
   if (d >= 0) {
     const Body& b = s.bodies[d];
-    v.world = b.world;
     v.lin_vel = b.lin_vel;
     v.ang_vel = b.ang_vel;
     v.inv_inertia_world = b.inv_inertia_world;
     v.flags = b.collision_flags;
   } else {
-    v.world = c.pose;
     v.lin_vel = splat(0.0f);
     v.ang_vel = splat(0.0f);
     v.inv_inertia_world = M3{splat(0.0f), splat(0.0f), splat(0.0f)};

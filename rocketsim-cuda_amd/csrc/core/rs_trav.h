@@ -160,8 +160,9 @@ __device__ inline bool trav_pooled(const StaticData& w, TravBatch& tb) {
                    __builtin_bit_cast(float, *wave_word(rec + 2)));
       box.max = v3(__builtin_bit_cast(float, *wave_word(rec + 3)), __builtin_bit_cast(float, *wave_word(rec + 4)),
                    __builtin_bit_cast(float, *wave_word(rec + 5)));
-      const WideNode* nodes =
-          (const WideNode*)(uintptr_t)(((uint64_t)*wave_word(rec + 7) << 32) | (uint64_t)*wave_word(rec + 6));
+      // the job's node array lives in the world (global memory): said so, its loads are global_ rather than flat ones
+      const RS_GLOBAL_AS WideNode* nodes =
+          (const RS_GLOBAL_AS WideNode*)(uintptr_t)(((uint64_t)*wave_word(rec + 7) << 32) | (uint64_t)*wave_word(rec + 6));
       nd = nodes[node];
     };
     if (i < n) fetch(i);
