@@ -501,8 +501,7 @@ RS_INLINE bool flat_tie_segment(V3 half, const SatPair* pr, V3* r0, V3* r1) {
 // One call per test; its four stages (the pair enumeration, the flat-tie segment, the penetration witness and the clipped
 // centroid) are inlined into it, each having this single caller: as calls they saved and restored the caller's
 // registers through private memory on every test.
-RS_NOINL bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
-  RS_PRIVATE(&box_trans); RS_PRIVATE(q); RS_PRIVATE(out);  // locals of the caller
+RS_INLINE bool box_triangle_sat_body(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
   V3 tmin = vmin(vmin(q[0], q[1]), q[2]), tmax = vmax(vmax(q[0], q[1]), q[2]);
   for (int ax = 0; ax < 3; ++ax) {  // pre-screen, X then Y then Z
     float lo = comp(tmin, ax), hi = comp(tmax, ax), h = comp(half, ax);
@@ -565,6 +564,16 @@ RS_NOINL bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, cons
   out->point_world = xform_point(box_trans, pb);
   out->distance = dist;
   return true;
+}
+// The test as a function of its own; its arguments are locals of the caller, and its pointers say so (scratch_
+// instructions rather than flat ones).
+RS_NOINL bool box_triangle_sat_in(const RS_PRIVATE_AS Aff* box_trans, V3 half, float margin, const RS_PRIVATE_AS V3* q,
+                                  float maxd, RS_PRIVATE_AS SatContact* out) {
+  return box_triangle_sat_body(*(const Aff*)box_trans, half, margin, (const V3*)q, maxd, (SatContact*)out);
+}
+RS_FN bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
+  return box_triangle_sat_in((const RS_PRIVATE_AS Aff*)&box_trans, half, margin, (const RS_PRIVATE_AS V3*)q, maxd,
+                             (RS_PRIVATE_AS SatContact*)out);
 }
 
 }  // namespace rs

@@ -382,8 +382,12 @@ RS_FN void solver_setup(Solver& sv, ArenaState& s, const StaticData& w, float dt
     const Manifold& m = s.manifolds[s.active[ai]];
     int d0 = dyn_slot(w, m.body0), d1 = dyn_slot(w, m.body1);
     int ia = solver_body_of(sv, s, w, d0, m.body0), ib = solver_body_of(sv, s, w, d1, m.body1);
-    V3 t0 = d0 >= 0 ? s.bodies[d0].world.t : w.bodies[m.body0].pose.t;
-    V3 t1 = d1 >= 0 ? s.bodies[d1].world.t : w.bodies[m.body1].pose.t;
+    // both candidates loaded, one picked: a conditional address would mix the state's and the world's memory into a
+    // generic pointer (flat loads)
+    const V3 s0 = s.bodies[d0 >= 0 ? d0 : 0].world.t, s1 = s.bodies[d1 >= 0 ? d1 : 0].world.t;
+    const V3 p0 = w.bodies[m.body0].pose.t, p1 = w.bodies[m.body1].pose.t;
+    V3 t0 = d0 >= 0 ? s0 : p0;
+    V3 t1 = d1 >= 0 ? s1 : p1;
     for (uint32_t pi = 0; pi < m.n_points; ++pi) {
       const ManifoldPoint& mp = m.pts[pi];
       V3 r1 = mp.world_on_a - t0, r2 = mp.world_on_b - t1;

@@ -18,41 +18,57 @@ constexpr int BANK_BODIES = 8;
 constexpr int BANK_BODY_WORDS = 16;
 constexpr int BANK_ROWS = 32;
 constexpr int BANK_WORDS = BANK_BODIES * BANK_BODY_WORDS + BANK_ROWS;
-// Both live in the same per-lane LDS slice (they are used in different phases), entry-major: word k of lane t of the
-// work-group is at k * blockDim + t.
+// Both live in the same per-lane LDS slice (they are used in different phases). Each wave of the work-group owns
+// 64 * LANE_LDS_WORDS words, entry-major: word k of lane l of wave v is at v * 64 * LANE_LDS_WORDS + k * 64 + l. The layout
+// does not depend on the work-group size (a kernel allocates whole waves), so the device functions never read it: they
+// need no dispatch-packet pointer, which every call would otherwise carry.
 constexpr int LANE_LDS_WORDS = TRAV_LDS > BANK_WORDS ? TRAV_LDS : BANK_WORDS;
-RS_HD inline size_t trav_lds_bytes(uint32_t block) { return (size_t)LANE_LDS_WORDS * block * sizeof(uint32_t); }
+constexpr uint32_t WAVE_LDS_WORDS = 64u * (uint32_t)LANE_LDS_WORDS;
+RS_HD inline size_t trav_lds_bytes(uint32_t block) {
+  return (size_t)WAVE_LDS_WORDS * ((block + 63u) / 64u) * sizeof(uint32_t);
+}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 extern __shared__ uint32_t rs_trav_lds[];
+// Word k of this lane's slice, and word k of lane `slot`'s slice of the same wave.
+__device__ inline uint32_t* lane_word(uint32_t k) {
+  return rs_trav_lds + ((threadIdx.x >> 6) * WAVE_LDS_WORDS + k * 64u + (threadIdx.x & 63u));
+}
+__device__ inline uint32_t* wave_lane_word(uint32_t slot, uint32_t k) {
+  return rs_trav_lds + ((threadIdx.x >> 6) * WAVE_LDS_WORDS + k * 64u + slot);
+}
 struct TravStack {
   uint32_t spill[TRAV_CAP - TRAV_LDS];
   int n;
   __device__ void push(uint32_t v) {
     if (n >= TRAV_CAP) return;
-    if (n < TRAV_LDS) rs_trav_lds[(uint32_t)n * blockDim.x + threadIdx.x] = v;
+    if (n < TRAV_LDS) *lane_word((uint32_t)n) = v;
     else spill[n - TRAV_LDS] = v;
     ++n;
   }
   __device__ uint32_t pop() {
     --n;
-    return n < TRAV_LDS ? rs_trav_lds[(uint32_t)n * blockDim.x + threadIdx.x] : spill[n - TRAV_LDS];
+    if (n < TRAV_LDS) return *lane_word((uint32_t)n);
+    // The deep branch reads an LDS word as well and picks: with a single load in each branch the compiler merges the two
+    // into one load through a generic pointer, a flat instruction on the common (LDS) path too.
+    const uint32_t deep = spill[n - TRAV_LDS], top = *lane_word(TRAV_LDS - 1);
+    return n < TRAV_LDS ? top : deep;
   }
 };
 // The solver's bank during the iterations: in LDS on the GPU, so that a row reading a body the previous row just updated,
 // or the impulse of the contact row a friction row is tied to, waits on LDS rather than on a private-memory round trip.
 // Fields 0-4 of a body are the deltas (dlin, dang, push, turn) and the inverse mass; word 15 is 1 for a dynamic body.
 struct DeltaBank {
-  __device__ static float* at(uint32_t word) { return reinterpret_cast<float*>(rs_trav_lds) + (word * blockDim.x + threadIdx.x); }
+  __device__ static float* at(uint32_t word) { return reinterpret_cast<float*>(lane_word(word)); }
   __device__ V3 get(int body, int field) const {
     const float* p = at((uint32_t)(body * BANK_BODY_WORDS + field * 3));
-    return V3{p[0], p[blockDim.x], p[2 * blockDim.x]};
+    return V3{p[0], p[64], p[128]};
   }
   __device__ void set(int body, int field, V3 v) {
     float* p = at((uint32_t)(body * BANK_BODY_WORDS + field * 3));
     p[0] = v.x;
-    p[blockDim.x] = v.y;
-    p[2 * blockDim.x] = v.z;
+    p[64] = v.y;
+    p[128] = v.z;
   }
   __device__ bool dynamic(int body) const { return *at((uint32_t)(body * BANK_BODY_WORDS + 15)) != 0.0f; }
   __device__ void set_dynamic(int body, bool d) { *at((uint32_t)(body * BANK_BODY_WORDS + 15)) = d ? 1.0f : 0.0f; }
@@ -62,9 +78,7 @@ struct DeltaBank {
 // Wave-wide staging through the same slices (the narrowphase's batched separating-axis tests, between the BVH walks): slot
 // r of a wave is the slice of the wave's lane r, so any lane of the wave can write or read any slot. LDS accesses of one
 // wave execute in order; the fences only keep the compiler from moving them across the hand-off points.
-__device__ inline float* wave_slot(uint32_t slot, uint32_t k) {
-  return reinterpret_cast<float*>(rs_trav_lds) + (k * blockDim.x + (threadIdx.x & ~63u) + slot);
-}
+__device__ inline float* wave_slot(uint32_t slot, uint32_t k) { return reinterpret_cast<float*>(wave_lane_word(slot, k)); }
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
