@@ -234,6 +234,33 @@ def regrouping_leg(rsim, dev, dw, model, states_host, controls, schedule, every,
     return out
 
 
+def cpu_checker(reference=True):
+    """The CPU build a configuration leg compares with: the reference's own C++ build when it was built with the tree
+    (oracle/_ref) and `reference` is set, else the oracle. Returns (name, step) with step(states, worlds, variant,
+    controls, threads) -> result words of one decision, the states stepped in place; the reference takes one StaticData
+    per call, so its arenas are stepped world by world (one call per world and decision: fine for a few worlds, slow for
+    the 112-world table, whose 1% sample is checked with the oracle)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    if reference and os.path.exists(orc.REF_LIB):
+        ref = orc.Reference()
+
+        def step(cpu, worlds, variant, controls, threads):
+            res = np.zeros(cpu.shape[0], np.uint32)
+            for p in np.unique(variant):
+                idx = np.nonzero(variant == p)[0]
+                sub = np.ascontiguousarray(cpu[idx])
+                res[idx] = ref.sim8(sub, worlds[p], np.ascontiguousarray(controls[idx]), threads=threads)
+                cpu[idx] = sub
+            return res
+        return "the reference's own C++ build", step
+    if not os.path.exists(orc.ORACLE_LIB):
+        orc.build(ref=False)
+    o = orc.Oracle()
+    return "the oracle", lambda cpu, worlds, variant, controls, threads: o.sim8_in(cpu, worlds, variant, controls, threads=threads)
+
+
 def open_play_leg(rsim, dev, decisions, block, threads):
     """16,384 open-play arenas over the seven hitbox-preset pairings (preset i vs preset i, arena a in pairing a mod 7),
     `decisions` plain per-launch decisions with per-arena worlds; every result word and final state compared with the
@@ -258,18 +285,19 @@ def open_play_leg(rsim, dev, decisions, block, threads):
     ms = dev.timer_end()
     res = [dev.read_words(r) for r in d_r]
     fin = dev.download_states(d_s)
-    o = orc.Oracle()
+    what, step = cpu_checker()
     cpu = states.copy()
     diffs = 0
     t0 = time.perf_counter()
     for k in range(decisions):
-        diffs += int((o.sim8_in(cpu, worlds, variant, controls[k], threads=threads) != res[k]).sum())
+        diffs += int((step(cpu, worlds, variant, controls[k], threads) != res[k]).sum())
     cpu_s = time.perf_counter() - t0
     diffs += int((cpu[:, :TICK_WORDS] != fin[:, :TICK_WORDS]).sum())
     stepped = stepped_ticks(res)
     return {"arenas": n, "decisions": decisions, "worlds": 7, "arena_ticks_per_s": stepped / (ms / 1000.0),
             "ms_per_decision": ms / decisions, "kernel": "k_sim8 (per-arena worlds)", "word_diffs_vs_cpu": diffs,
-            "compared": "every arena, every decision", "cpu_arena_ticks_per_s": stepped / cpu_s, "cpu_threads": threads}
+            "compared": "every arena, every decision, with %s" % what, "cpu_arena_ticks_per_s": stepped / cpu_s,
+            "cpu_threads": threads}
 
 
 def scale_leg(rsim, dev, model, decisions, block, threads, chunk=50):
@@ -327,7 +355,7 @@ def scale_leg(rsim, dev, model, decisions, block, threads, chunk=50):
     reset = (lambda: g.reset(d_init)) if g is not None else (lambda: dev.copy_states(d_s, d_init))
     ms, wall = timed_graph(dev, rec, reset)
     stepped = 0
-    o = orc.Oracle()
+    what, step = cpu_checker(reference=False)
     cpu = np.ascontiguousarray(states[sample])
     vs = variant[sample]
     diffs = 0
@@ -335,7 +363,7 @@ def scale_leg(rsim, dev, model, decisions, block, threads, chunk=50):
         r = dev.read_words(d_r[c]).reshape(chunk, n)
         stepped += stepped_ticks(list(r))
         for k in range(chunk):
-            diffs += int((o.sim8_in(cpu, worlds, vs, np.ascontiguousarray(ctl_sample[c][k]), threads=threads) != r[k, sample]).sum())
+            diffs += int((step(cpu, worlds, vs, np.ascontiguousarray(ctl_sample[c][k]), threads) != r[k, sample]).sum())
         diffs += int((cpu[:, :TICK_WORDS] != dev.download_states(d_g[c])[:, :TICK_WORDS]).sum())
     achieved = B_TICK * stepped / (ms / 1000.0)
     return {"arenas": n, "decisions": chunks * chunk, "worlds": int(worlds.shape[0]), "arena_ticks_per_s": stepped / (ms / 1000.0),
@@ -343,8 +371,8 @@ def scale_leg(rsim, dev, model, decisions, block, threads, chunk=50):
             "kernel": "k_run_queue, %d decisions per launch, %s, one HIP graph" % (
                 chunk, "arenas regrouped by the cost model (sorted) before each launch" if g is not None else "plain order"),
             "hbm_frac_at_b_tick": achieved / HBM_PEAK,
-            "word_diffs_vs_cpu": diffs, "compared": "a 1%% sample (%d arenas): result words of every decision, states every %d"
-            % (sample.size, chunk), "state_gigabytes": n * 11136 / 1e9}
+            "word_diffs_vs_cpu": diffs, "compared": "a 1%% sample (%d arenas): result words of every decision, states every %d, "
+            "with %s" % (sample.size, chunk, what), "state_gigabytes": n * 11136 / 1e9}
 
 
 def main():
