@@ -55,11 +55,13 @@ RS_FN Aabb child_world_box(const Np& c, uint32_t b) {
 }
 
 // ---- persistent manifolds (persistent_manifold.cuh) ---------------------------------------------------------------------------
-RS_FN Manifold manifold_new(uint32_t b0, uint32_t b1) {
-  Manifold m = Manifold{};
+// A fresh manifold built in place: the header only. Its point slots past n_points are never read (points are added at
+// n_points, the four-point replacement runs only when all four are written, a merge or an insert reads n_points of them),
+// and an insert writes the unused slots of the pool's manifold as zeros, so no 79-word zero-fill and copy per pair.
+RS_FN void manifold_init(Manifold& m, uint32_t b0, uint32_t b1) {
   m.body0 = b0;
   m.body1 = b1;
-  return m;
+  m.n_points = 0;
 }
 RS_FN float breaking_threshold(const Np& c, const Manifold& m) {
   return fmin_rs(c.w.bodies[m.body0].contact_breaking_threshold, c.w.bodies[m.body1].contact_breaking_threshold);

@@ -285,11 +285,11 @@ RS_FN bool alg_box_box(Np& c, uint32_t ba, uint32_t bb, Manifold* out) {  // :37
   Obb o2 = Obb{tb.t, tb.m, add_s(c2.box_half_extents, c2.box_margin)};
   BoxHit hit;
   if (!box_box_sat(o1, transpose(ta.m), o2, &hit)) return false;
-  Manifold m = manifold_new(ba, bb);
+  Manifold& m = *out;
+  manifold_init(m, ba, bb);
   box_box_contacts(c, o1, ta.m, o2, tb.m, hit, m, ba, bb);
   if (m.n_points == 0) return false;
   refresh_points(c, m, ba, bb);
-  *out = m;
   return true;
 }
 
@@ -305,7 +305,8 @@ RS_FN bool alg_sphere_obb(Np& c, uint32_t sphere, uint32_t box, Manifold* out) {
   V3 closest = vclamp(local, -ext, ext);
   V3 delta = local - closest;
   float d2 = length_sq(delta);
-  Manifold m = manifold_new(sphere, box);
+  Manifold& m = *out;
+  manifold_init(m, sphere, box);
   float isect = radius + bc.box_margin;
   float cdist = isect + breaking_threshold(c, m);
   if (d2 > cdist * cdist) return false;
@@ -339,9 +340,7 @@ RS_FN bool alg_sphere_obb(Np& c, uint32_t sphere, uint32_t box, Manifold* out) {
   V3 p_box = xform_point(ct, closest);
   add_contact(c, m, sphere, box, n_box, p_box + n_box * bc.box_margin, dist - isect, -1);
   refresh_points(c, m, sphere, box);
-  if (m.n_points == 0) return false;
-  *out = m;
-  return true;
+  return m.n_points != 0;
 }
 
 // ---- convex-plane (convex_plane_collision_alg.cuh) ------------------------------------------------------------------------------
@@ -366,12 +365,12 @@ RS_FN bool convex_plane(Np& c, uint32_t obj, uint32_t plane_obj, const Aff& wt, 
   }
   V3 vip = xform_point(convex_in_plane, vtx);
   float distance = dot(pn, vip);
-  Manifold m = manifold_new(obj, plane_obj);
+  Manifold& m = *out;
+  manifold_init(m, obj, plane_obj);
   if (distance >= breaking_threshold(c, m)) return false;
   V3 proj = vip - splat(distance) * pn;
   add_contact(c, m, obj, plane_obj, pn, proj + pt.t, distance, -1);
   refresh_points(c, m, obj, plane_obj);
-  *out = m;
   return true;
 }
 
@@ -640,7 +639,8 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
   const BodyConst& bc = c.w.bodies[car];
   CarMeshFrame f = car_mesh_frame(c, car, mesh_body);
   const Tri* tris = c.w.tris + mesh->tri_first;
-  Manifold m = manifold_new(car, mesh_body);
+  Manifold& m = *out;
+  manifold_init(m, car, mesh_body);
   bool has_m = false;
   const SatJob* job = nullptr;
   if (c.sb)
@@ -673,18 +673,38 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
   }
   if (!has_m) return false;
   refresh_points(c, m, car, mesh_body);
-  if (m.n_points == 0) return false;
-  *out = m;
-  return true;
+  return m.n_points != 0;
 }
 
 // ---- the dispatcher (collision_dispatcher.cuh) -----------------------------------------------------------------------------------
+// Appends a fresh manifold to the pool: its header and points, the unused point slots written as zeros (the whole
+// manifold, as a copy of a zero-filled one would write it).
 RS_FN int insert_manifold(ArenaState& s, const Manifold& m) {
   if (s.n_manifolds >= (uint32_t)MAX_MANIFOLDS) {
     s.overflow[OVF_MANIFOLDS] += 1;
     return -1;
   }
-  s.manifolds[s.n_manifolds] = m;
+  Manifold& d = s.manifolds[s.n_manifolds];
+  d.body0 = m.body0;
+  d.body1 = m.body1;
+  d.n_points = m.n_points;
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
+    if (i < m.n_points) d.pts[i] = m.pts[i];
+    else d.pts[i] = ManifoldPoint{};
+  }
+  return (int)s.n_manifolds++;
+}
+// Appends an empty manifold of bodies b0, b1 (all points zero).
+RS_FN int insert_manifold_empty(ArenaState& s, uint32_t b0, uint32_t b1) {
+  if (s.n_manifolds >= (uint32_t)MAX_MANIFOLDS) {
+    s.overflow[OVF_MANIFOLDS] += 1;
+    return -1;
+  }
+  Manifold& d = s.manifolds[s.n_manifolds];
+  d.body0 = b0;
+  d.body1 = b1;
+  d.n_points = 0;
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) d.pts[i] = ManifoldPoint{};
   return (int)s.n_manifolds++;
 }
 RS_FN void push_active(ArenaState& s, uint32_t idx) {
@@ -735,7 +755,7 @@ RS_FN void near_tail(Np& c, uint32_t a, uint32_t b, int cached, bool has_fresh, 
   } else {
     uint32_t f, g;
     empty_manifold_order(c, a, b, &f, &g);
-    act = insert_manifold(s, manifold_new(f, g));
+    act = insert_manifold_empty(s, f, g);
   }
   if (act >= 0) push_active(s, (uint32_t)act);
 }
@@ -753,13 +773,14 @@ RS_FN void near_callback(Np& c, uint32_t a, uint32_t b) {
   }
   if ((ka == k::SHAPE_SPHERE && kb == k::SHAPE_MESH) || (ka == k::SHAPE_MESH && kb == k::SHAPE_SPHERE)) {
     uint32_t ball = ka == k::SHAPE_SPHERE ? a : b, mesh_b = ka == k::SHAPE_SPHERE ? b : a;
-    int idx = cached >= 0 ? cached : insert_manifold(s, manifold_new(ball, mesh_b));
+    int idx = cached >= 0 ? cached : insert_manifold_empty(s, ball, mesh_b);
     if (idx < 0) return;
     sphere_mesh(c, ball, mesh_b, s.manifolds[idx]);
     push_active(s, (uint32_t)idx);
     return;
   }
-  Manifold fresh = Manifold{};
+  Manifold fresh;  // built in place by the algorithm that reports a contact (manifold_init); read only then
+  fresh.n_points = 0;
   bool has_fresh = false;
   uint32_t ko = c.w.bodies[other].shape_kind;
   if (compound_pair && ko == k::SHAPE_PLANE) has_fresh = convex_plane(c, comp_b, other, child_world(c, comp_b), true, &fresh);
