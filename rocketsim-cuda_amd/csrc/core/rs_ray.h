@@ -310,7 +310,7 @@ RS_FN void compound_rays(const BodyConst& bc, BodyRays& br, const RayPacket& rp)
   float org[3][4], inv[3][4];
   packet_pos_dir(rp, org, inv);
   uint32_t mask = slab_mask(org, inv, box, br.frac);
-  for (int i = 0; i < 4; ++i)
+  RS_UNROLL for (int i = 0; i < 4; ++i)
     if (mask & (1u << i)) compound_ray(bc, br, rp.src[i], rp.dst[i], i);
 }
 RS_FN void sphere_rays(float radius, BodyRays& br, const RayPacket& rp) {  // sphere_shape.rs (:261-337)
@@ -400,7 +400,7 @@ RS_FN void plane_rays(const Plane& pl, BodyRays& br, const RayPacket& rp) {  // 
     if (t[l] >= 0.0f && t[l] < dist[l]) tmask |= 1u << l;
   }
   uint32_t hm = rmask & (amask & tmask);
-  for (int l = 0; l < 4; ++l)
+  RS_UNROLL for (int l = 0; l < 4; ++l)
     if (hm & (1u << l)) body_report(br, n, t[l] / dist[l], l);
 }
 
@@ -419,7 +419,7 @@ RS_FN void mesh_rays(const StaticData& w, const MeshInfo& mesh, BodyRays& br, Ra
       uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
       if (mask != 0) {
         const Tri& tri = w.tris[mesh.tri_first + leaf.tri];
-        for (int i = 0; i < 4; ++i)
+        RS_UNROLL for (int i = 0; i < 4; ++i)
           if (mask & (1u << i)) ray_triangle(br, tri, &rp.lambda_max[i], i);
       }
       continue;
@@ -446,7 +446,7 @@ RS_FN void mesh_rays_leaves(const StaticData& w, const MeshInfo& mesh, BodyRays&
     const Tri tri = w.tris[mesh.tri_first + leaf.tri];
     uint32_t mask = slab_mask(org, inv, leaf.bounds, rp.lambda_max);
     if (mask != 0) {
-      for (int i = 0; i < 4; ++i)
+      RS_UNROLL for (int i = 0; i < 4; ++i)
         if (mask & (1u << i)) ray_triangle(br, tri, &rp.lambda_max[i], i);
     }
   }
@@ -656,7 +656,7 @@ RS_FN void cast_wheel_rays(const ArenaState& s, const StaticData& w, const V3* f
   ClosestHits hits;
   wheel_hits_init(hits, from, to, ignore);
   ray_tests(s, w, hits, prep.cand[ci], prep.ncand[ci], prep.tb, prep.first_job[ci], prep.end_job[ci]);
-  for (int i = 0; i < 4; ++i) {
+  RS_UNROLL for (int i = 0; i < 4; ++i) {
     out[i].some = 0;
     out[i].point = splat(0.0f);
     out[i].normal = splat(0.0f);
