@@ -196,7 +196,7 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
   float cache_angle = 0.0f;
   V3 cache_axle = splat(0.0f);
   int in_contact = 0;
-  for (int i = 0; i < NUM_WHEELS; ++i) {
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) {
     Wheel& wh = car.wheels[i];
     if (!hits[i].some) {
       wheel_reset(wh);
@@ -225,12 +225,12 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
   }
   if (in_contact < 3)
     for (int i = 0; i < NUM_WHEELS; ++i) car.wheels[i].engine_force = car.wheels[i].engine_force / 4.0f;
-  for (int i = 0; i < NUM_WHEELS; ++i) wheel_suspension(car.wheels[i], cc.wheels[i], s.bodies[cd], cbc, chassis_t, dt);
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_suspension(car.wheels[i], cc.wheels[i], s.bodies[cd], cbc, chassis_t, dt);
   {
     BodyView chassis = body_view(s, w, chassis_body);
-    for (int i = 0; i < NUM_WHEELS; ++i) wheel_friction_impulse(car.wheels[i], chassis, cbc.inv_inertia_local, dt, friction_scale);
+    RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_friction_impulse(car.wheels[i], chassis, cbc.inv_inertia_local, dt, friction_scale);
   }
-  for (int i = 0; i < NUM_WHEELS; ++i) wheel_apply_friction(car.wheels[i], s.bodies[cd], cbc, ct, dt);
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_apply_friction(car.wheels[i], s.bodies[cd], cbc, ct, dt);
 }
 
 // ---- auto-flip roll sign (car_pre_tick.cuh:23-72) ------------------------------------------------------------------------------
