@@ -396,13 +396,14 @@ RS_FN void tri_edge_box_edge(V3 tp, V3 td, float ta, V3 b0, V3 b1, V3* o_tri, V3
   *o_tri = tp + td * s;
   *o_box = b0 + d2 * t;
 }
-// :960 the 47 candidate pairs: 3 clamped vertices, 8 box corners against the triangle, 3 x 12 edge pairs
-RS_INLINE void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr) {
-  RS_PRIVATE(q); RS_PRIVATE(edges); RS_PRIVATE(pr);  // locals of the caller
-  int n = 0;
+// :960 the 47 candidate pairs, handed to emit in order: 3 clamped vertices, 8 box corners against the triangle, 3 x 12
+// edge pairs
+template <class Emit>
+RS_INLINE void enum_pairs_each(V3 half, const V3 q[3], const V3 edges[3], Emit emit) {
+  RS_PRIVATE(q); RS_PRIVATE(edges);  // locals of the caller
   for (int i = 0; i < 3; ++i) {
     V3 pb = clamp_to_box(q[i], half);
-    pr[n++] = SatPair{pb, q[i], length_sq(pb - q[i])};
+    emit(SatPair{pb, q[i], length_sq(pb - q[i])});
   }
   V3 ab = edges[0], ac = q[2] - q[0];
   for (int ix = 0; ix < 2; ++ix)
@@ -410,7 +411,7 @@ RS_INLINE void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr
       for (int iz = 0; iz < 2; ++iz) {
         V3 corner = v3((ix ? 1.0f : -1.0f) * half.x, (iy ? 1.0f : -1.0f) * half.y, (iz ? 1.0f : -1.0f) * half.z);
         V3 pt = closest_on_tri(corner, q, ab, ac);
-        pr[n++] = SatPair{corner, pt, length_sq(corner - pt)};
+        emit(SatPair{corner, pt, length_sq(corner - pt)});
       }
   const V3 c[8] = {v3(fneg(half.x), fneg(half.y), fneg(half.z)), v3(half.x, fneg(half.y), fneg(half.z)),
                    v3(half.x, half.y, fneg(half.z)),             v3(fneg(half.x), half.y, fneg(half.z)),
@@ -422,14 +423,24 @@ RS_INLINE void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr
     for (int kk = 0; kk < 12; ++kk) {
       V3 pt, pb;
       tri_edge_box_edge(q[ti], edges[ti], ta, c[be[kk][0]], c[be[kk][1]], &pt, &pb);
-      pr[n++] = SatPair{pb, pt, length_sq(pb - pt)};
+      emit(SatPair{pb, pt, length_sq(pb - pt)});
     }
   }
 }
-RS_FN SatPair first_min_pair(const SatPair* pr) {  // strict <: the first pair wins ties
-  SatPair best = pr[0];
-  for (int i = 1; i < SAT_NPAIRS; ++i)
-    if (pr[i].d2 < best.d2) best = pr[i];
+RS_INLINE void enum_pairs(V3 half, const V3 q[3], const V3 edges[3], SatPair* pr) {  // all 47, stored
+  RS_PRIVATE(pr);  // a local of the caller
+  int n = 0;
+  enum_pairs_each(half, q, edges, [&](const SatPair& p) { pr[n++] = p; });
+}
+// The first pair of least distance (strict <: the first pair wins ties), found while the pairs are generated, so the 47
+// pairs need not be stored for it.
+RS_INLINE SatPair first_min_pair(V3 half, const V3 q[3], const V3 edges[3]) {
+  SatPair best;
+  bool have = false;
+  enum_pairs_each(half, q, edges, [&](const SatPair& p) {
+    if (!have || p.d2 < best.d2) best = p;
+    have = true;
+  });
   return best;
 }
 // :1096 flat_tie_segment: the triangle-side endpoints of a flat contact interval, or false
@@ -523,17 +534,19 @@ RS_INLINE bool box_triangle_sat_body(const Aff& box_trans, V3 half, float margin
   if (sw.has_sep) {
     V3 nw = normalize_or_zero(box_trans.m * sw.best_sep_axis);
     if (length_sq(nw) < 0.5f) return false;
-    SatPair pr[SAT_NPAIRS];
-    enum_pairs(half, q, edges, pr);
-    SatPair first = first_min_pair(pr);
+    const SatPair first = first_min_pair(half, q, edges);
     if (length_sq(first.pbox - first.ptri) > 1e-24f) {
       float plen = length(first.pbox - first.ptri);
       V3 pb = first.ptri;
       bool snapped = false;
       V3 s0, s1;
-      if (plen - margin < 0.0f && flat_tie_segment(half, pr, &s0, &s1)) {
-        pb = closest_on_segment(centroid, s0, s1);
-        snapped = true;
+      if (plen - margin < 0.0f) {  // the flat-tie segment reads every pair: they are generated again, the same way
+        SatPair pr[SAT_NPAIRS];
+        enum_pairs(half, q, edges, pr);
+        if (flat_tie_segment(half, pr, &s0, &s1)) {
+          pb = closest_on_segment(centroid, s0, s1);
+          snapped = true;
+        }
       }
       float dist = snapped ? length(clamp_to_box(pb, half) - pb) - margin : plen - margin;
       if (dist >= maxd) return false;
