@@ -76,8 +76,10 @@ RS_FN float res_area(V3 p, V3 a, V3 b, V3 c) { return length_sq(cross(p - a, b -
 RS_FN uint32_t replace_slot(const Manifold& m, const ManifoldPoint& np) {  // sort_cached_points (:81-117)
   uint32_t deepest = MAX_POINTS;
   float maxpen = np.distance;
-  for (uint32_t i = 0; i < m.n_points; ++i)
-    if (m.pts[i].distance < maxpen) {
+  const uint32_t n = m.n_points;
+  RS_UNROLL
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i)  // the point loops run over the four slots with a guard, so the
+    if (i < n && m.pts[i].distance < maxpen) {       // points' loads need not wait for each other
       deepest = i;
       maxpen = m.pts[i].distance;
     }
@@ -98,7 +100,10 @@ RS_FN int cache_entry(const Np& c, const Manifold& m, const ManifoldPoint& np) {
   float cbt = breaking_threshold(c, m);
   float shortest = cbt * cbt;
   int nearest = -1;
-  for (uint32_t i = 0; i < m.n_points; ++i) {
+  const uint32_t n = m.n_points;
+  RS_UNROLL
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
+    if (i >= n) continue;
     float d2 = length_sq(m.pts[i].local_a - np.local_a);
     if (d2 < shortest) {
       shortest = d2;
@@ -139,14 +144,19 @@ RS_FN void merge_points(const Np& c, Manifold& m, const Manifold& o) {  // :196-
 RS_FN void refresh_points(const Np& c, Manifold& m, uint32_t b0, uint32_t b1) {  // :208-213 with persistent_manifold.rs:312
   if (m.n_points == 0) return;
   Aff ta = body_trans(c, b0), tb = body_trans(c, b1);
-  for (uint32_t i = 0; i < m.n_points; ++i) {
+  const uint32_t n = m.n_points;
+  RS_UNROLL
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
+    if (i >= n) continue;
     ManifoldPoint& p = m.pts[i];
     p.world_on_a = xform_point(ta, p.local_a);
     p.world_on_b = xform_point(tb, p.local_b);
     p.distance = dot(p.world_on_a - p.world_on_b, p.normal_on_b);
   }
   float cbt = breaking_threshold(c, m), cbt2 = cbt * cbt;
-  for (int i = (int)m.n_points - 1; i >= 0; --i) {
+  RS_UNROLL
+  for (int i = MAX_POINTS - 1; i >= 0; --i) {  // from the last point down, over the points present at the start
+    if (i >= (int)n) continue;
     ManifoldPoint p = m.pts[i];
     if (p.distance > cbt) {
       point_swap_remove(m, (uint32_t)i);
