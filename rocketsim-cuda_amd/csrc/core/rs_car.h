@@ -106,49 +106,60 @@ RS_FN void wheel_friction_curves(Wheel& wh, const BodyView& chassis, V3 chassis_
   wh.lat_friction = lat_f;
   wh.long_friction = long_f;
 }
-// wheel_info.cuh:130-171 calc_friction_impulses / update_friction_impulse
-RS_FN void wheel_friction_impulse(Wheel& wh, const BodyView& chassis, V3 inv_inertia_local, float dt, float friction_scale) {
-  if (!wh.ray.some) return;
-  V3 cn = wh.ray.v.contact_normal, cp = wh.ray.v.contact_point;
-  V3 axle = normalize_or_zero(wh.axle_dir);
+// What the impulse passes of the vehicle update read of a wheel (its ray result, axle, drive and friction), read for all
+// four wheels at once, after the last write to any of it: no wheel's loads then wait behind another wheel's branch.
+struct WheelContact {
+  bool some;
+  V3 cn, cp, axle, vcp;
+  float susp_len, clipped, srv, pushback, engine, brake, lng, lat;
+};
+RS_FN WheelContact wheel_contact(const Wheel& wh) {
+  const RaycastInfo& ri = wh.ray.v;
+  return WheelContact{wh.ray.some != 0, ri.contact_normal, ri.contact_point, wh.axle_dir, wh.vel_at_contact_point,
+                      ri.suspension_length, ri.clipped_inv_contact_dot_suspension, ri.suspension_relative_vel,
+                      wh.extra_pushback, wh.engine_force, wh.brake, wh.long_friction, wh.lat_friction};
+}
+// wheel_info.cuh:130-171 calc_friction_impulses / update_friction_impulse: the wheel's friction impulse (a wheel out of
+// contact keeps its stored one)
+RS_FN V3 wheel_friction_impulse(const WheelContact& wc, const BodyView& chassis, V3 inv_inertia_local, float dt, float friction_scale) {
+  V3 cn = wc.cn, cp = wc.cp;
+  V3 axle = normalize_or_zero(wc.axle);
   V3 fwd = normalize_or_zero(cross(cn, axle));
   float side = resolve_bilateral_ground(chassis, inv_inertia_local, cp, axle);
   float rolling;
-  if (wh.engine_force == 0.0f) {
-    if (wh.brake == 0.0f) {
+  if (wc.engine == 0.0f) {
+    if (wc.brake == 0.0f) {
       rolling = 0.0f;
     } else {
-      float rel_vel = dot(wh.vel_at_contact_point, fwd);
+      float rel_vel = dot(wc.vcp, fwd);
       if (dt > 1.0f / 80.0f) {
         float th = 0.8f - 1.0f / (dt * 150.0f);
         if (fabs_(rel_vel) < th) rel_vel = 0.0f;
       }
-      rolling = fclamp_rs(fneg(rel_vel) * 113.73963f, fneg(wh.brake), wh.brake);
+      rolling = fclamp_rs(fneg(rel_vel) * 113.73963f, fneg(wc.brake), wc.brake);
     }
   } else {
-    rolling = fneg(wh.engine_force) / friction_scale;
+    rolling = fneg(wc.engine) / friction_scale;
   }
-  V3 total = fwd * rolling * wh.long_friction + axle * side * wh.lat_friction;
-  wh.ray.v.impulse = total * friction_scale;
+  V3 total = fwd * rolling * wc.lng + axle * side * wc.lat;
+  return total * friction_scale;
 }
 // wheel_info.cuh:174-191 update_suspension
-RS_FN void wheel_suspension(Wheel& wh, const WheelConst& wc, Body& cb, const BodyConst& cbc, V3 chassis_t, float dt) {
-  if (!wh.ray.some) return;
-  const RaycastInfo& ri = wh.ray.v;
-  float force = ((wc.suspension_rest_length - ri.suspension_length) * k::SUSPENSION_STIFFNESS) * ri.clipped_inv_contact_dot_suspension;
-  float damp = ri.suspension_relative_vel < 0.0f ? k::WHEELS_DAMPING_COMPRESSION : k::WHEELS_DAMPING_RELAXATION;
-  float f = (force - damp * ri.suspension_relative_vel) * wc.suspension_force_scale;
+RS_FN void wheel_suspension(const WheelContact& wcn, const WheelConst& wc, Body& cb, const BodyConst& cbc, V3 chassis_t, float dt) {
+  if (!wcn.some) return;
+  float force = ((wc.suspension_rest_length - wcn.susp_len) * k::SUSPENSION_STIFFNESS) * wcn.clipped;
+  float damp = wcn.srv < 0.0f ? k::WHEELS_DAMPING_COMPRESSION : k::WHEELS_DAMPING_RELAXATION;
+  float f = (force - damp * wcn.srv) * wc.suspension_force_scale;
   if (f <= 0.0f) return;
-  float scale = f * dt + wh.extra_pushback;
-  add_impulse(cb, cbc, IMP_LINEAR_REL_POS, ri.contact_normal * scale, ri.contact_point - chassis_t, true, false);
+  float scale = f * dt + wcn.pushback;
+  add_impulse(cb, cbc, IMP_LINEAR_REL_POS, wcn.cn * scale, wcn.cp - chassis_t, true, false);
 }
 // wheel_info.cuh:194-201 apply_friction_impulses
-RS_FN void wheel_apply_friction(const Wheel& wh, Body& cb, const BodyConst& cbc, const Aff& ct, float dt) {
-  if (!wh.ray.some) return;
-  const RaycastInfo& ri = wh.ray.v;
-  V3 off = ri.contact_point - ct.t;
+RS_FN void wheel_apply_friction(const WheelContact& wcn, V3 impulse, Body& cb, const BodyConst& cbc, const Aff& ct, float dt) {
+  if (!wcn.some) return;
+  V3 off = wcn.cp - ct.t;
   float up_dot = dot(ct.m.c2, off);
-  add_impulse(cb, cbc, IMP_LINEAR_REL_POS, ri.impulse * dt, off - ct.m.c2 * up_dot, true, false);
+  add_impulse(cb, cbc, IMP_LINEAR_REL_POS, impulse * dt, off - ct.m.c2 * up_dot, true, false);
 }
 
 // ---- the vehicle (vehicle_rl.cuh) --------------------------------------------------------------------------------------------
@@ -225,12 +236,21 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
   }
   if (in_contact < 3)
     for (int i = 0; i < NUM_WHEELS; ++i) car.wheels[i].engine_force = car.wheels[i].engine_force / 4.0f;
-  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_suspension(car.wheels[i], cc.wheels[i], s.bodies[cd], cbc, chassis_t, dt);
+  WheelContact wcn[NUM_WHEELS];
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wcn[i] = wheel_contact(car.wheels[i]);
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_suspension(wcn[i], cc.wheels[i], s.bodies[cd], cbc, chassis_t, dt);
+  V3 imp[NUM_WHEELS];
   {
     BodyView chassis = body_view(s, w, chassis_body);
-    RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_friction_impulse(car.wheels[i], chassis, cbc.inv_inertia_local, dt, friction_scale);
+    RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) {
+      imp[i] = car.wheels[i].ray.v.impulse;
+      if (wcn[i].some) {
+        imp[i] = wheel_friction_impulse(wcn[i], chassis, cbc.inv_inertia_local, dt, friction_scale);
+        car.wheels[i].ray.v.impulse = imp[i];
+      }
+    }
   }
-  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_apply_friction(car.wheels[i], s.bodies[cd], cbc, ct, dt);
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) wheel_apply_friction(wcn[i], imp[i], s.bodies[cd], cbc, ct, dt);
 }
 
 // ---- auto-flip roll sign (car_pre_tick.cuh:23-72) ------------------------------------------------------------------------------
