@@ -207,14 +207,20 @@ RS_FN void vehicle_update(ArenaState& s, const StaticData& w, int ci, float dt, 
   float cache_angle = 0.0f;
   V3 cache_axle = splat(0.0f);
   int in_contact = 0;
+  // The chassis and the wheels' grounds read before the loop: no wheel's effects reach the chassis (its rays ignore
+  // it), and a ground changes in the loop only when it is a dynamic body an earlier wheel pushed, so a dynamic ground is
+  // read again at its wheel.
+  const BodyView chassis = body_view(s, w, chassis_body);
+  BodyView grounds[NUM_WHEELS];
+  RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) grounds[i] = body_view(s, w, hits[i].some ? hits[i].body : chassis_body);
   RS_UNROLL for (int i = 0; i < NUM_WHEELS; ++i) {
     Wheel& wh = car.wheels[i];
     if (!hits[i].some) {
       wheel_reset(wh);
       continue;
     }
-    BodyView chassis = body_view(s, w, chassis_body);
-    BodyView ground = body_view(s, w, hits[i].body);
+    BodyView ground = grounds[i];
+    if (!view_static(ground)) ground = body_view(s, w, hits[i].body);
     ++in_contact;
     bool front = i < 2;
     V3 axle;
