@@ -272,14 +272,30 @@ RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_g
       t.n = kept;
     }
   }
-  for (int ci = 0; ci < NUM_CARS; ++ci) {  // discovery through the cylinder
+  // discovery through the cylinder. Both cars' box screens first, in one pass over the pads (each pad's box read once):
+  // a screen reads only the car's position and the pads' boxes, which the discovery does not change.
+  bool look[NUM_CARS];
+  V3 cpos[NUM_CARS];
+  for (int ci = 0; ci < NUM_CARS; ++ci) {
     const Car& car = s.cars[ci];
-    if (car.is_demoed || car.pos.z > w.max_pad_z || touching[ci] >= 0) continue;
-    uint64_t in_box = 0;
+    cpos[ci] = car.pos;
+    look[ci] = !(car.is_demoed || car.pos.z > w.max_pad_z || touching[ci] >= 0);
+  }
+  uint64_t in_boxes[NUM_CARS] = {};
+  if (look[0] || look[1]) {
+    static_assert(NUM_CARS == 2, "the screens test two cars");
     RS_UNROLL
-    for (int i = 0; i < NUM_PADS; ++i)
-      in_box |= (uint64_t)(all3(cmple(w.pads[i].aabb.min, car.pos)) && all3(cmpge(w.pads[i].aabb.max, car.pos))) << i;
-    for (uint64_t m = in_box; m != 0; m &= m - 1) {
+    for (int i = 0; i < NUM_PADS; ++i) {
+      const Aabb b = w.pads[i].aabb;
+      RS_UNROLL
+      for (int ci = 0; ci < NUM_CARS; ++ci)
+        in_boxes[ci] |= (uint64_t)(all3(cmple(b.min, cpos[ci])) && all3(cmpge(b.max, cpos[ci]))) << i;
+    }
+  }
+  for (int ci = 0; ci < NUM_CARS; ++ci) {
+    if (!look[ci]) continue;
+    const Car& car = s.cars[ci];
+    for (uint64_t m = in_boxes[ci]; m != 0; m &= m - 1) {
       const int i = __builtin_ctzll(m);
       const PadConst& pad = w.pads[i];
       if (!pad_cyl(pad.pos, pad.cyl_radius, car.pos)) continue;
