@@ -356,7 +356,7 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
 
 // P3a (collision_world.cuh:34-50): the dynamic bodies' swept boxes (current and predicted transform, each grown by the
 // contact breaking threshold) into the broadphase.
-RS_FN void phase_p3a(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p3a(ArenaState& s, const StaticData& w) {
   const V3 cbt = splat(k::CONTACT_BREAKING_THRESHOLD);
   for (uint32_t d = 0; d < (uint32_t)NUM_DYN; ++d) {
     const BodyConst& bc = w.bodies[w.dyn_body[d]];

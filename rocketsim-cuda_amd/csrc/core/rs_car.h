@@ -655,7 +655,7 @@ RS_FN void car_flip_zdamp(ArenaState& s, const StaticData& w, int ci) {  // appl
   Body& rb = s.bodies[dyn_slot(w, w.cars[ci].body_idx)];
   if (rb.lin_vel.z < 0.0f || car.flip_time < k::FLIP_Z_DAMP_END) rb.lin_vel.z = rb.lin_vel.z * (1.0f - k::FLIP_Z_DAMP_120);
 }
-RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci, const WheelRayPrep& prep) {  // car/base.rs pre_tick_update (car_pre_tick.cuh:638-703)
+RS_INLINE void car_pre_tick(ArenaState& s, const StaticData& w, int ci, const WheelRayPrep& prep) {  // car/base.rs pre_tick_update (car_pre_tick.cuh:638-703)
   Car& car = s.cars[ci];
   const CarConst& cc = w.cars[ci];
   Body& rb = s.bodies[dyn_slot(w, cc.body_idx)];
@@ -709,8 +709,12 @@ RS_FN void car_pre_tick(ArenaState& s, const StaticData& w, int ci, const WheelR
 // A car's pre-tick as a function of its own, called once per car: the boundary lets the register allocator treat the
 // car's update apart from the rest of the phase (measured faster than inlining it). Its state and world pointers carry
 // their address spaces, so its accesses compile to scratch_ and global_ instructions.
-RS_NOINL void car_pre_tick_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp, int ci,
+// UW: every lane steps in the same world (its pointer is then made wave-uniform).
+template <bool UW>
+RS_NOINL void car_pre_tick_in(RS_PRIVATE_AS ArenaState* sp, const RS_CONST_AS StaticData* wp, int ci,
                               const RS_PRIVATE_AS WheelRayPrep* pp) {
+  if (UW) wp = rs_uniform_ptr(wp);
+  RS_UNIFORM_INDEX(ci);
   car_pre_tick(*(ArenaState*)sp, *(const StaticData*)wp, ci, *(const WheelRayPrep*)pp);
 }
 // Both cars' wheel-ray screens and mesh walks, before either car's pre-tick (exact: see WheelRayPrep). A demoed car casts
@@ -735,13 +739,14 @@ RS_FN void wheel_rays_prepare(const ArenaState& s, const StaticData& w, WheelRay
   }
   trav_run(w, prep.tb);
 }
-RS_FN void phase_p1(ArenaState& s, const StaticData& w) {
+template <bool UW = false>
+RS_INLINE void phase_p1(ArenaState& s, const StaticData& w) {
   for (int ci = 0; ci < NUM_CARS; ++ci) car_flip_zdamp(s, w, ci);
   WheelRayPrep prep;
   wheel_rays_prepare(s, w, prep);
   for (int ci = 0; ci < NUM_CARS; ++ci) {
     uint32_t was = s.cars[ci].is_on_ground;
-    car_pre_tick_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w, ci, (const RS_PRIVATE_AS WheelRayPrep*)&prep);
+    car_pre_tick_in<UW>((RS_PRIVATE_AS ArenaState*)&s, (const RS_CONST_AS StaticData*)&w, ci, (const RS_PRIVATE_AS WheelRayPrep*)&prep);
     if (!was && s.cars[ci].is_on_ground) s.events |= 1u << (EV_LANDED + ci);
   }
 }

@@ -30,15 +30,39 @@
 // StaticData through pointers qualified with these compiles their accesses to scratch_* / global_* instructions; through
 // generic pointers they would be flat instructions, which wait on both the vector-memory and the LDS/scalar counters and
 // need 64-bit VGPR addresses. Empty on the host.
-// RS_PRIVATE(p) states the same for a pointer parameter that always points into the lane's private memory.
+// RS_PRIVATE(p) states the same for a pointer parameter that always points into the lane's private memory. The StaticData
+// is passed as constant memory (RS_CONST_AS): no kernel writes it, and the compiler may then read it with scalar loads
+// wherever the address is the same in every lane (it cannot prove that of global memory inside a non-kernel function).
 #if defined(__HIP_DEVICE_COMPILE__)
 #define RS_PRIVATE_AS __attribute__((address_space(5)))
 #define RS_GLOBAL_AS __attribute__((address_space(1)))
+#define RS_CONST_AS __attribute__((address_space(4)))
 #define RS_PRIVATE(p) __builtin_assume(__builtin_amdgcn_is_private((const void*)(p)))
 #else
 #define RS_PRIVATE_AS
 #define RS_GLOBAL_AS
+#define RS_CONST_AS
 #define RS_PRIVATE(p) ((void)0)
+#endif
+
+// Wave-uniform values across a call. A non-inlined function receives its arguments in vector registers, so the compiler
+// treats them as lane-varying: a world pointer that every lane shares would then be read with vector loads, and a phase or
+// car index would steer divergent branches. RS_UNIFORM_INDEX(i) states that the active lanes hold the same index (true of
+// the phase and car indices: every lane runs the same phases and cars in the same order); rs_uniform_ptr(p) does the same
+// for a pointer, for callers whose lanes all step in one world, so reads at uniform offsets become scalar loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RS_UNIFORM_INDEX(i) ((i) = __builtin_amdgcn_readfirstlane(i))
+template <class T>
+__device__ inline T* rs_uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+#else
+#define RS_UNIFORM_INDEX(i) ((void)0)
+template <class T>
+inline T* rs_uniform_ptr(T* p) { return p; }
 #endif
 
 #if defined(__clang__)

@@ -840,11 +840,15 @@ RS_INLINE uint32_t phase_p3b_body(ArenaState& s, const StaticData& w) {
 // The phase as a function of its own whose state and world pointers carry their address spaces: its accesses compile to
 // scratch_ and global_ instructions (through generic pointers they would be flat ones, which wait on both the vector-memory
 // and the LDS counters and need 64-bit addresses).
-RS_NOINL uint32_t phase_p3b_in(RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp) {
+// UW: every lane steps in the same world (its pointer is then made wave-uniform).
+template <bool UW>
+RS_NOINL uint32_t phase_p3b_in(RS_PRIVATE_AS ArenaState* sp, const RS_CONST_AS StaticData* wp) {
+  if (UW) wp = rs_uniform_ptr(wp);
   return phase_p3b_body(*(ArenaState*)sp, *(const StaticData*)wp);
 }
+template <bool UW = false>
 RS_FN uint32_t phase_p3b(ArenaState& s, const StaticData& w) {
-  return phase_p3b_in((RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w);
+  return phase_p3b_in<UW>((RS_PRIVATE_AS ArenaState*)&s, (const RS_CONST_AS StaticData*)&w);
 }
 
 }  // namespace rs

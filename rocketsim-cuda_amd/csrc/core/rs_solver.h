@@ -45,7 +45,7 @@ RS_FN void hoare_sort(uint32_t* idx, int n, const int* keys) {
     if (lo < j) { slo[sp] = lo; shi[sp] = j; ++sp; }
   }
 }
-RS_FN void phase_p4(ArenaState& s, const StaticData& w) {  // :100-150
+RS_INLINE void phase_p4(ArenaState& s, const StaticData& w) {  // :100-150
   const int n = (int)s.n_active;
   int parent[NUM_DYN];
   for (int d = 0; d < NUM_DYN; ++d) parent[d] = d;
@@ -518,7 +518,7 @@ RS_FN void solver_finish(Solver& sv, ArenaState& s, float dt) {  // :271-301
     }
   }
 }
-RS_FN void phase_p5(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p5(ArenaState& s, const StaticData& w) {
   const float dt = 1.0f / 120.0f;
   Solver sv;
   sv.n_bodies = sv.n_contact = sv.n_friction = sv.n_special = 0;

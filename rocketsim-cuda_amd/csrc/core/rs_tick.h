@@ -10,7 +10,7 @@
 namespace rs {
 
 // ---- P0 (arena_base.cuh:21-38) ---------------------------------------------------------------------------------------------------
-RS_FN void phase_p0(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p0(ArenaState& s, const StaticData& w) {
   s.events = 0;
   for (int d = 0; d < NUM_DYN; ++d) clear_accum(s.bodies[d]);
   for (int ci = 0; ci < NUM_CARS; ++ci) {
@@ -28,7 +28,7 @@ RS_FN void phase_p0(ArenaState& s, const StaticData& w) {
 }
 
 // ---- P2 / P6 (discrete_dynamics_world.cuh) ---------------------------------------------------------------------------------------
-RS_FN void phase_p2(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p2(ArenaState& s, const StaticData& w) {
   const float dt = 1.0f / 120.0f;
   V3 g = w.gravity_bt * dt;
   for (int d = 0; d < NUM_DYN; ++d)
@@ -43,7 +43,7 @@ RS_FN void phase_p2(ArenaState& s, const StaticData& w) {
     b.interp_world = predict_trans(b, dt);
   }
 }
-RS_FN void phase_p6(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p6(ArenaState& s, const StaticData& w) {
   const float dt = 1.0f / 120.0f;
   for (int d = 0; d < NUM_DYN; ++d) {
     Body& b = s.bodies[d];
@@ -136,7 +136,7 @@ RS_FN void car_car_hit(ArenaState& s, const StaticData& w, int c1, int c2, const
     s.events |= 1u << ((demo ? EV_DEMO : EV_BUMP) + vic);
   }
 }
-RS_FN void phase_p7(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p7(ArenaState& s, const StaticData& w) {
   s.events |= s.fold_events;
   for (int ci = 0; ci < NUM_CARS; ++ci)
     if (s.fold_events & (1u << (EV_CAR_HIT_WORLD + ci))) {
@@ -332,7 +332,7 @@ RS_FN void boost_pads_tick(ArenaState& s, const StaticData& w) {  // boost_pad_g
       if (touching[cj] == i) win[cj] = nw;
   }
 }
-RS_FN void phase_p8(ArenaState& s, const StaticData& w) {
+RS_INLINE void phase_p8(ArenaState& s, const StaticData& w) {
   for (int ci = 0; ci < NUM_CARS; ++ci) {
     car_post_tick(s.cars[ci], s.bodies[1 + ci]);
     car_finish_tick(s.cars[ci], s.bodies[1 + ci]);
@@ -352,15 +352,19 @@ constexpr int NUM_PHASES = 10;
 enum PhaseStatus : int { PHASE_OK = 0, PHASE_UNIMPLEMENTED = -1, PHASE_BAD_INDEX = -2 };
 
 // Phase k of one tick (0 P0, 1 P1, 2 P2, 3 P3a, 4 P3b, 5 P4, 6 P5, 7 P6, 8 P7, 9 P8): state at boundary k to boundary k + 1.
-RS_NOINL int run_phase_in(int kk, RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_AS StaticData* wp) {
+// UW: every lane steps in the same world (its pointer is then made wave-uniform); k is the same in every lane.
+template <bool UW>
+RS_NOINL int run_phase_in(int kk, RS_PRIVATE_AS ArenaState* sp, const RS_CONST_AS StaticData* wp) {
+  if (UW) wp = rs_uniform_ptr(wp);
+  RS_UNIFORM_INDEX(kk);
   ArenaState& s = *(ArenaState*)sp;
   const StaticData& w = *(const StaticData*)wp;
   switch (kk) {
     case 0: phase_p0(s, w); return PHASE_OK;
-    case 1: phase_p1(s, w); return PHASE_OK;
+    case 1: phase_p1<UW>(s, w); return PHASE_OK;
     case 2: phase_p2(s, w); return PHASE_OK;
     case 3: phase_p3a(s, w); return PHASE_OK;
-    case 4: return phase_p3b(s, w) == 0 ? PHASE_OK : PHASE_UNIMPLEMENTED;
+    case 4: return phase_p3b<UW>(s, w) == 0 ? PHASE_OK : PHASE_UNIMPLEMENTED;
     case 5: phase_p4(s, w); return PHASE_OK;
     case 6: phase_p5(s, w); return PHASE_OK;
     case 7: phase_p6(s, w); return PHASE_OK;
@@ -369,15 +373,18 @@ RS_NOINL int run_phase_in(int kk, RS_PRIVATE_AS ArenaState* sp, const RS_GLOBAL_
     default: return PHASE_BAD_INDEX;
   }
 }
-// Every caller holds its arena in a lane-private copy and the StaticData in global memory.
+// Every caller holds its arena in a lane-private copy and the StaticData in global memory. UW (uniform world): the
+// caller's lanes all step in the same world.
+template <bool UW = false>
 RS_FN int run_phase(int kk, ArenaState& s, const StaticData& w) {
-  return run_phase_in(kk, (RS_PRIVATE_AS ArenaState*)&s, (const RS_GLOBAL_AS StaticData*)&w);
+  return run_phase_in<UW>(kk, (RS_PRIVATE_AS ArenaState*)&s, (const RS_CONST_AS StaticData*)&w);
 }
 // One tick (base.rs step_tick): the ten phases in order.
+template <bool UW = false>
 RS_FN int step_tick(ArenaState& s, const StaticData& w) {
   int st = PHASE_OK;
   for (int kk = 0; kk < NUM_PHASES; ++kk)
-    if (run_phase(kk, s, w) != PHASE_OK) st = PHASE_UNIMPLEMENTED;
+    if (run_phase<UW>(kk, s, w) != PHASE_OK) st = PHASE_UNIMPLEMENTED;
   return st;
 }
 RS_FN bool ball_scored(const ArenaState& s, const StaticData& w) {  // base.rs is_ball_scored, Soccar
@@ -394,12 +401,13 @@ RS_FN bool any_overflow(const ArenaState& s) {
 // after the tick on which the ball is first scored, so *s is the goal state then (the reference steps the remaining ticks
 // and discards them; stopping is the same result). after_tick(t, s) sees the state after each tick up to the goal.
 // Returns the result word: the scored tick (0 for none) | SIM8_UNIMPLEMENTED | SIM8_OVERFLOW.
-template <class AfterTick>
+// UW: as for step_tick.
+template <bool UW = false, class AfterTick>
 RS_FN uint32_t run_decision(ArenaState& s, const StaticData& w, const Controls* ctl, uint32_t n, AfterTick after_tick) {
   uint32_t unimpl = 0, scored = 0;
   for (uint32_t t = 1; t <= n; ++t) {
     for (int ci = 0; ci < NUM_CARS; ++ci) s.cars[ci].controls = ctl[ci];
-    if (step_tick(s, w) != PHASE_OK) unimpl = SIM8_UNIMPLEMENTED;
+    if (step_tick<UW>(s, w) != PHASE_OK) unimpl = SIM8_UNIMPLEMENTED;
     after_tick(t, s);
     if (ball_scored(s, w)) {
       scored = t;

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_phase(int k, uint32_t* buf, const
   if (a >= n) return;
   ArenaState s;
   load_all(s, buf, a);
-  status[a] = run_phase(k, s, *w);
+  status[a] = run_phase<true>(k, s, *w);
   store_all(buf, a, s);
 }
 __global__ void __launch_bounds__(MAX_BLOCK) k_step_tick(uint32_t* buf, const StaticData* __restrict__ w, uint32_t n,
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_step_tick(uint32_t* buf, const St
   if (a >= n) return;
   ArenaState s;
   load_tick(s, buf, a);
-  status[a] = step_tick(s, *w);
+  status[a] = step_tick<true>(s, *w);
   store_tick(buf, a, s);
 }
 
@@ -80,8 +80,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_sim8(Sim8Args p) {
   load_tick(s, p.buf, a);
   Controls c[NUM_CARS];
   for (int i = 0; i < NUM_CARS; ++i) c[i] = p.ctl[(size_t)a * NUM_CARS + i];
-  if (TRACE) p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, TraceOut{p.trace, aosoa_words(p.n), a});
-  else p.result[a] = run_decision(s, w, c, TICKS_PER_DECISION, NoTrace{});
+  // without a variant array every lane steps in worlds[0]: the uniform-world instantiation
+  if (TRACE) p.result[a] = run_decision<!VARIANTS>(s, w, c, TICKS_PER_DECISION, TraceOut{p.trace, aosoa_words(p.n), a});
+  else p.result[a] = run_decision<!VARIANTS>(s, w, c, TICKS_PER_DECISION, NoTrace{});
   store_tick(p.buf, a, s);
   if (p.cycles) p.cycles[a] = wall_clock64() - t0;  // a wave's lanes share its clock: with one lane per work-group
                                                      // (block 1) this is the arena's own cost
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_sim8_ticks(Sim8Args p, uint32_t f
   const uint32_t last = first + count - 1;
   for (uint32_t t = first; t <= last; ++t) {
     for (int ci = 0; ci < NUM_CARS; ++ci) s.cars[ci].controls = c[ci];
-    if (step_tick(s, w) != PHASE_OK) word |= SIM8_UNIMPLEMENTED;
+    if (step_tick<!VARIANTS>(s, w) != PHASE_OK) word |= SIM8_UNIMPLEMENTED;
     if (ball_scored(s, w)) {
       word |= t;
       break;
@@ -129,6 +130,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_sim8_ticks(Sim8Args p, uint32_t f
 // (one copy-in, one copy-out), with the controls of decision d at ctl[d * n * NUM_CARS ...] and its result words at
 // result[d * n ...]. Decision d + 1 starts where decision d ended (the goal state after a goal), as consecutive sim8
 // launches do; the mid-tick tail is cleared between decisions, as the tick-boundary copy clears it.
+template <bool UW>  // UW: no variant array, every lane in worlds[0]
 __device__ inline void run_arena(const Sim8Args& p, uint32_t a, uint32_t decisions) {
   uint32_t v = p.variant ? p.variant[a] : 0u;
   if (v >= p.nworlds) {
@@ -145,14 +147,15 @@ __device__ inline void run_arena(const Sim8Args& p, uint32_t a, uint32_t decisio
     Controls c[NUM_CARS];
     const Controls* cd = p.ctl + ((size_t)d * p.n + a) * NUM_CARS;
     for (int i = 0; i < NUM_CARS; ++i) c[i] = cd[i];
-    p.result[(size_t)d * p.n + a] = run_decision(s, w, c, TICKS_PER_DECISION, NoTrace{});
+    p.result[(size_t)d * p.n + a] = run_decision<UW>(s, w, c, TICKS_PER_DECISION, NoTrace{});
   }
   store_tick(p.buf, a, s);
 }
 __global__ void __launch_bounds__(MAX_BLOCK) k_run(Sim8Args p, uint32_t decisions) {
   uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= p.n) return;
-  run_arena(p, a, decisions);
+  if (p.variant) run_arena<false>(p, a, decisions);
+  else run_arena<true>(p, a, decisions);
 }
 
 // k_run with dynamic scheduling: persistent work-groups whose waves each take the next AoSoA block of 64 arenas from a
@@ -171,7 +174,10 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_run_queue(Sim8Args p, uint32_t de
     if (u >= nblocks) break;
     const uint32_t blk = order ? order[u] : u;
     const uint32_t a = blk * AOSOA_LANES + lane;
-    if (blk < nblocks && a < p.n) run_arena(p, a, decisions);
+    if (blk < nblocks && a < p.n) {
+      if (p.variant) run_arena<false>(p, a, decisions);
+      else run_arena<true>(p, a, decisions);
+    }
   }
 }
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
