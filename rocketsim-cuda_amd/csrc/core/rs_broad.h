@@ -356,7 +356,14 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
 
 // P3a (collision_world.cuh:34-50): the dynamic bodies' swept boxes (current and predicted transform, each grown by the
 // contact breaking threshold) into the broadphase.
+// The sweep's arrays (the handles' edge indices and the three edge lists, contiguous in the state) warmed before the
+// sweep, whose accesses are chains of dependent reads.
+RS_FN void sap_warm(const ArenaState& s) {
+  static_assert(offsetof(ArenaState, edges) == offsetof(ArenaState, handle_edges) + sizeof(s.handle_edges), "contiguous");
+  warm_words<(int)((sizeof(s.handle_edges) + sizeof(s.edges)) / 4)>(&s.handle_edges[0]);
+}
 RS_INLINE void phase_p3a(ArenaState& s, const StaticData& w) {
+  sap_warm(s);
   const V3 cbt = splat(k::CONTACT_BREAKING_THRESHOLD);
   for (uint32_t d = 0; d < (uint32_t)NUM_DYN; ++d) {
     const BodyConst& bc = w.bodies[w.dyn_body[d]];
