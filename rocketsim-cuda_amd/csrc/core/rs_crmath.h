@@ -120,8 +120,12 @@ RS_FN uint64_t bits_at(const uint64_t p[4], int pos) {  // bits [pos, pos + 64) 
   return w;
 }
 
-// |x| = q pi/2 + r (mod 2 pi) for finite |x| >= pi/4: returns q mod 4, r in radians as a double-double, |r| <= pi/4 + tiny.
-RS_FN int reduce_pio2(float x, DD* r) {
+// |x| = q pi/2 + r (mod 2 pi) for finite |x| >= pi/4: q mod 4, and r in radians as a double-double, |r| <= pi/4 + tiny.
+// Out of line on the device (as are the accurate paths below): the stepper inlines every call site of the transcendentals,
+// and a copy of these long integer and double-double sequences at each one would bloat the hot phases' code for branches
+// that are rarely or never taken there.
+struct Reduced { DD r; int q; };
+RS_NOINL Reduced reduce_pio2(float x) {
   uint32_t u = fbits(x) & 0x7FFFFFFFu;
   uint64_t m = (u & 0x7FFFFFu) | 0x800000u;
   int E = (int)(u >> 23) - 150;  // |x| = m 2^E
@@ -164,8 +168,7 @@ RS_FN int reduce_pio2(float x, DD* r) {
   }
   DD f = fast_two_sum(d1, d2);
   if (negative) f = neg(f);
-  *r = mul(f, DD{crt::pio2_hi, crt::pio2_lo});
-  return q;
+  return Reduced{mul(f, DD{crt::pio2_hi, crt::pio2_lo}), q};
 }
 
 // ---- sin, cos, tan ----------------------------------------------------------------------------------------------------------
@@ -204,6 +207,21 @@ RS_FN DD cos_dd(DD r) {
   return p;  // 1/0! - z/2! + z^2/4! ...
 }
 
+// The accurate path of trig_cr: the double-double value of sin, cos or tan at q pi/2 + r, sign-flipped when `flip`, rounded.
+RS_NOINL float trig_accurate(DD r, int q, int which, bool flip) {
+  DD v;
+  if (which == 2) {
+    DD s = sin_dd(r), c = cos_dd(r);
+    v = (q & 1) ? neg(div(c, s)) : div(s, c);
+  } else {
+    int k = which == 0 ? q : q + 1;
+    v = (k & 1) ? cos_dd(r) : sin_dd(r);
+    if (k & 2) v = neg(v);
+  }
+  if (flip) v = neg(v);
+  return dd_round_f32(v.hi, v.lo);
+}
+
 // which: 0 sin, 1 cos, 2 tan
 RS_FN float trig_cr(float x, int which) {
   uint32_t ax = fbits(x) & 0x7FFFFFFFu;
@@ -215,7 +233,9 @@ RS_FN float trig_cr(float x, int which) {
   if (ax < 0x3F490FDBu) {  // |x| < pi/4 (the float below pi/4's float)
     r = DD{(double)ffrom(ax), 0.0};
   } else {
-    q = reduce_pio2(x, &r);
+    Reduced red = reduce_pio2(x);
+    r = red.r;
+    q = red.q;
   }
   // fast path
   double y;
@@ -232,18 +252,7 @@ RS_FN float trig_cr(float x, int which) {
     float f = (float)y;
     return flip ? -f : f;
   }
-  // accurate path
-  DD v;
-  if (which == 2) {
-    DD s = sin_dd(r), c = cos_dd(r);
-    v = (q & 1) ? neg(div(c, s)) : div(s, c);
-  } else {
-    int k = which == 0 ? q : q + 1;
-    v = (k & 1) ? cos_dd(r) : sin_dd(r);
-    if (k & 2) v = neg(v);
-  }
-  if (flip) v = neg(v);
-  return dd_round_f32(v.hi, v.lo);
+  return trig_accurate(r, q, which, flip);
 }
 RS_FN float sinf_cr(float x) { return trig_cr(x, 0); }
 RS_FN float cosf_cr(float x) { return trig_cr(x, 1); }
@@ -280,6 +289,15 @@ RS_FN DD quot_dd(double num, double den) {
   return fast_two_sum(q, rem / den);
 }
 
+// The accurate path of atan2f_cr: atan(num/den) in double-double, reflected as the caller's fast path does, rounded.
+RS_NOINL float atan2f_accurate(double num, double den, bool swap, bool xneg, bool yneg) {
+  DD v = atan_dd(quot_dd(num, den));
+  if (swap) v = sub(DD{crt::pio2_hi, crt::pio2_lo}, v);
+  if (xneg) v = sub(DD{crt::pi_hi, crt::pi_lo}, v);
+  if (yneg) v = neg(v);
+  return dd_round_f32(v.hi, v.lo);
+}
+
 RS_FN float atan2f_cr(float y, float x) {
   uint32_t uy = fbits(y), ux = fbits(x), ay = uy & 0x7FFFFFFFu, ax = ux & 0x7FFFFFFFu;
   bool yneg = (uy >> 31) != 0, xneg = (ux >> 31) != 0;
@@ -302,11 +320,7 @@ RS_FN float atan2f_cr(float y, float x) {
     float f = (float)res;
     return yneg ? -f : f;
   }
-  DD v = atan_dd(quot_dd(num, den));
-  if (swap) v = sub(DD{crt::pio2_hi, crt::pio2_lo}, v);
-  if (xneg) v = sub(DD{crt::pi_hi, crt::pi_lo}, v);
-  if (yneg) v = neg(v);
-  return dd_round_f32(v.hi, v.lo);
+  return atan2f_accurate(num, den, swap, xneg, yneg);
 }
 
 // atan2 of binary64 arguments in double-double, one final rounding to binary64 (the reference's atan2_64 is likewise a
