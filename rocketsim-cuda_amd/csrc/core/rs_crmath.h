@@ -325,7 +325,7 @@ RS_FN float atan2f_cr(float y, float x) {
 
 // atan2 of binary64 arguments in double-double, one final rounding to binary64 (the reference's atan2_64 is likewise a
 // double-double method, not correctly rounded: libm_cr.cuh:941-944). Used by the auto-flip roll sign, on binary32 values.
-RS_FN double atan2_64(double y, double x) {
+RS_NOINL double atan2_64(double y, double x) {
   uint64_t uy = dbits(y), ux = dbits(x), ay = uy & 0x7FFFFFFFFFFFFFFFull, ax = ux & 0x7FFFFFFFFFFFFFFFull;
   const uint64_t INF = 0x7FF0000000000000ull;
   bool yneg = (uy >> 63) != 0, xneg = (ux >> 63) != 0;
@@ -402,7 +402,7 @@ RS_FN void decompose_f64(uint64_t u, uint64_t* m, int* e) {  // |v| = m 2^e, m i
 }
 
 // sqrt(x^2 + y^2), correctly rounded: the sum of squares exactly in integers, its square root with remainder.
-RS_FN double hypot64(double x, double y) {
+RS_NOINL double hypot64(double x, double y) {
   uint64_t ux = dbits(x) & 0x7FFFFFFFFFFFFFFFull, uy = dbits(y) & 0x7FFFFFFFFFFFFFFFull;
   const uint64_t INF = 0x7FF0000000000000ull;
   if (ux == INF || uy == INF) return dfrom(INF);
@@ -561,7 +561,7 @@ RS_FN bool pow_is_exact_midpoint(float x, float y, double M) {
   return (num % ((int64_t)1 << k)) == 0 && num / ((int64_t)1 << k) == Me;
 }
 
-RS_FN float powf_cr(float x, float y) {
+RS_NOINL float powf_cr(float x, float y) {
   uint32_t ux = fbits(x), uy = fbits(y), ax = ux & 0x7FFFFFFFu, ay = uy & 0x7FFFFFFFu;
   bool xneg = (ux >> 31) != 0, yneg = (uy >> 31) != 0;
   if (ax == 0x3F800000u) {  // |x| = 1
