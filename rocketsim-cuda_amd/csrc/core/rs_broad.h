@@ -99,97 +99,174 @@ RS_FN void remove_pair(ArenaState& s, const StaticData& w, uint32_t ha, uint32_t
 }
 
 // ---- sweep and prune (axis_sweep.cuh) --------------------------------------------------------------------------------------
+// The sweep's working copy. During P3a the sorts read and write the handles' edge indices (six u16 per handle, 72 words)
+// and one axis's edge list at a time (48 words) in a copy: on the GPU the lane's LDS slice, which no other phase uses
+// between P1's wheel-ray walks and P3b's narrowphase walks; on the host a local array. A sort step is then a chain of LDS
+// reads and writes, where in private memory it was a chain of dependent round trips that also waited on the step's own
+// stores. The edge indices are copied in at the start of P3a and written back at its end for the handles whose indices
+// changed; an axis's edges are copied in when that axis's sort starts and written back over the slots the sorts wrote.
+// Indices are masked to the copy's extent (a malformed state stays within its own lane's words).
+constexpr uint32_t SAP_HE_WORDS = 3u * NUM_HANDLES;
+constexpr uint32_t SAP_WORDS = SAP_HE_WORDS + NUM_EDGES;
+static_assert(sizeof(HandleEdges) == 12 && sizeof(Edge) == 4 && sizeof(((ArenaState*)nullptr)->handle_edges) == 4 * SAP_HE_WORDS,
+              "HandleEdges is six u16 in three words, an Edge one word");
+static_assert(NUM_HANDLES <= 32 && SAP_HE_WORDS + 64u <= (uint32_t)LANE_LDS_WORDS, "masked indices stay in the lane's slice");
+struct SapCopy {
+#if !defined(__HIP_DEVICE_COMPILE__)
+  uint32_t words[SAP_HE_WORDS + 64u];
+#endif
+  uint32_t he_dirty;  // bit h: handle h's edge indices were written
+  uint32_t lo, hi;    // the current axis's edge slots written, lo > hi for none
+  RS_FN uint32_t* word(uint32_t k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return lane_word(k);
+#else
+    return &words[k];
+#endif
+  }
+  // u16 j of handle h's HandleEdges: min[a] is j = a, max[a] is j = 3 + a
+  RS_FN uint16_t* he(uint32_t h, uint32_t j) { return reinterpret_cast<uint16_t*>(word((h & 31u) * 3u + (j >> 1))) + (j & 1u); }
+  RS_FN uint32_t he_get(uint32_t h, uint32_t j) { return *he(h, j); }
+  RS_FN void he_set(uint32_t h, uint32_t j, uint32_t v) {
+    *he(h, j) = (uint16_t)v;
+    he_dirty |= 1u << (h & 31u);
+  }
+  RS_FN Edge edge(uint32_t i) { return __builtin_bit_cast(Edge, *word(SAP_HE_WORDS + (i & 63u))); }
+  RS_FN void set_edge(uint32_t i, Edge e) { *word(SAP_HE_WORDS + (i & 63u)) = __builtin_bit_cast(uint32_t, e); }
+  RS_FN void wrote(uint32_t a, uint32_t b) {  // edge slots [a, b]
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+};
+RS_FN void sap_copy_in(const ArenaState& s, SapCopy& c) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&s.handle_edges[0]);
+  uint32_t v[SAP_HE_WORDS];
+  RS_UNROLL
+  for (uint32_t k = 0; k < SAP_HE_WORDS; ++k) v[k] = src[k];
+  RS_UNROLL
+  for (uint32_t k = 0; k < SAP_HE_WORDS; ++k) *c.word(k) = v[k];
+  c.he_dirty = 0;
+}
+RS_FN void sap_copy_out(ArenaState& s, SapCopy& c) {
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&s.handle_edges[0]);
+  RS_UNROLL
+  for (uint32_t h = 0; h < (uint32_t)NUM_HANDLES; ++h)
+    if (c.he_dirty & (1u << h)) {
+      const uint32_t a = *c.word(3u * h), b = *c.word(3u * h + 1u), d = *c.word(3u * h + 2u);
+      dst[3u * h] = a;
+      dst[3u * h + 1u] = b;
+      dst[3u * h + 2u] = d;
+    }
+}
+RS_FN void sap_axis_in(const ArenaState& s, SapCopy& c, int ax) {
+  Edge v[NUM_EDGES];
+  RS_UNROLL
+  for (int i = 0; i < NUM_EDGES; ++i) v[i] = s.edges[ax][i];
+  RS_UNROLL
+  for (int i = 0; i < NUM_EDGES; ++i) c.set_edge((uint32_t)i, v[i]);
+  c.lo = (uint32_t)NUM_EDGES;
+  c.hi = 0;
+}
+RS_FN void sap_axis_out(ArenaState& s, SapCopy& c, int ax) {
+  const uint32_t hi = c.hi < (uint32_t)NUM_EDGES ? c.hi : (uint32_t)NUM_EDGES - 1u;
+  for (uint32_t i = c.lo; i <= hi; ++i) s.edges[ax][i] = c.edge(i);
+}
+
 RS_FN bool edge_max(Edge e) { return (e.pos & 1u) != 0; }
 RS_FN bool overlap_2d(const ArenaState& s, uint32_t ha, uint32_t hb, int axis) {  // :28-34, touching counts
   int a1 = (1 << axis) & 3, a2 = (1 << a1) & 3;
   const HandleEdges &a = s.handle_edges[ha], &b = s.handle_edges[hb];
   return !(a.max[a1] < b.min[a1] || b.max[a1] < a.min[a1] || a.max[a2] < b.min[a2] || b.max[a2] < a.min[a2]);
 }
+RS_FN bool overlap_2d(SapCopy& c, uint32_t ha, uint32_t hb, int axis) {  // the same test on the working copy
+  uint32_t a1 = (1u << axis) & 3u, a2 = (1u << a1) & 3u;
+  return !(c.he_get(ha, 3u + a1) < c.he_get(hb, a1) || c.he_get(hb, 3u + a1) < c.he_get(ha, a1) ||
+           c.he_get(ha, 3u + a2) < c.he_get(hb, a2) || c.he_get(hb, 3u + a2) < c.he_get(ha, a2));
+}
 // The four incremental sorts (axis_sweep.cuh:49-127): the moved edge bubbles past its neighbours, each passed neighbour's
 // edge index is adjusted, and a min edge passing a max edge (or the reverse) adds or removes the pair per direction.
 // The moving edge is held in registers and written once at its final slot, and its own edge index is adjusted once by
 // the number of slots passed: nothing inside the loop reads either (the overlap tests read the other two axes, the pair
-// and manifold updates no edges), so this is the reference's sequence of swaps, with a neighbour's load per step as the
-// only memory on the loop's critical path.
-RS_FN void sort_min_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
-  Edge* e = s.edges[ax];
-  const Edge cur = e[i];
+// and manifold updates no edges), so this is the reference's sequence of swaps.
+RS_FN void sort_min_down(ArenaState& s, const StaticData& w, SapCopy& c, int ax, uint32_t i) {
+  const Edge cur = c.edge(i);
   const uint32_t mv = cur.handle, i0 = i;
   while (i > 0) {
-    const Edge prev = e[i - 1];
+    const Edge prev = c.edge(i - 1);
     if (!(cur.pos < prev.pos)) break;
     if (edge_max(prev)) {
-      if (overlap_2d(s, mv, prev.handle, ax)) add_pair(s, w, mv, prev.handle);
-      s.handle_edges[prev.handle].max[ax] += 1;
+      if (overlap_2d(c, mv, prev.handle, ax)) add_pair(s, w, mv, prev.handle);
+      c.he_set(prev.handle, 3u + ax, c.he_get(prev.handle, 3u + ax) + 1u);
     } else {
-      s.handle_edges[prev.handle].min[ax] += 1;
+      c.he_set(prev.handle, ax, c.he_get(prev.handle, ax) + 1u);
     }
-    e[i] = prev;
+    c.set_edge(i, prev);
     --i;
   }
   if (i == i0) return;
-  e[i] = cur;
-  s.handle_edges[mv].min[ax] = (uint16_t)(s.handle_edges[mv].min[ax] - (uint16_t)(i0 - i));
+  c.set_edge(i, cur);
+  c.wrote(i, i0);
+  c.he_set(mv, ax, c.he_get(mv, ax) - (i0 - i));
 }
-RS_FN void sort_max_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
-  Edge* e = s.edges[ax];
-  const Edge cur = e[i];
+RS_FN void sort_max_up(ArenaState& s, const StaticData& w, SapCopy& c, int ax, uint32_t i) {
+  const Edge cur = c.edge(i);
   const uint32_t mv = cur.handle, i0 = i;
   while (i + 1 < (uint32_t)NUM_EDGES) {
-    const Edge next = e[i + 1];
+    const Edge next = c.edge(i + 1);
     if (!(next.handle != 0 && cur.pos >= next.pos)) break;
     if (!edge_max(next)) {
-      if (overlap_2d(s, mv, next.handle, ax)) add_pair(s, w, mv, next.handle);
-      s.handle_edges[next.handle].min[ax] -= 1;
+      if (overlap_2d(c, mv, next.handle, ax)) add_pair(s, w, mv, next.handle);
+      c.he_set(next.handle, ax, c.he_get(next.handle, ax) - 1u);
     } else {
-      s.handle_edges[next.handle].max[ax] -= 1;
+      c.he_set(next.handle, 3u + ax, c.he_get(next.handle, 3u + ax) - 1u);
     }
-    e[i] = next;
+    c.set_edge(i, next);
     ++i;
   }
   if (i == i0) return;
-  e[i] = cur;
-  s.handle_edges[mv].max[ax] = (uint16_t)(s.handle_edges[mv].max[ax] + (uint16_t)(i - i0));
+  c.set_edge(i, cur);
+  c.wrote(i0, i);
+  c.he_set(mv, 3u + ax, c.he_get(mv, 3u + ax) + (i - i0));
 }
-RS_FN void sort_min_up(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
-  Edge* e = s.edges[ax];
-  const Edge cur = e[i];
+RS_FN void sort_min_up(ArenaState& s, const StaticData& w, SapCopy& c, int ax, uint32_t i) {
+  const Edge cur = c.edge(i);
   const uint32_t mv = cur.handle, i0 = i;
   while (i + 1 < (uint32_t)NUM_EDGES) {
-    const Edge next = e[i + 1];
+    const Edge next = c.edge(i + 1);
     if (!(next.handle != 0 && cur.pos >= next.pos)) break;
     if (edge_max(next)) {
-      if (overlap_2d(s, mv, next.handle, ax)) remove_pair(s, w, mv, next.handle);
-      s.handle_edges[next.handle].max[ax] -= 1;
+      if (overlap_2d(c, mv, next.handle, ax)) remove_pair(s, w, mv, next.handle);
+      c.he_set(next.handle, 3u + ax, c.he_get(next.handle, 3u + ax) - 1u);
     } else {
-      s.handle_edges[next.handle].min[ax] -= 1;
+      c.he_set(next.handle, ax, c.he_get(next.handle, ax) - 1u);
     }
-    e[i] = next;
+    c.set_edge(i, next);
     ++i;
   }
   if (i == i0) return;
-  e[i] = cur;
-  s.handle_edges[mv].min[ax] = (uint16_t)(s.handle_edges[mv].min[ax] + (uint16_t)(i - i0));
+  c.set_edge(i, cur);
+  c.wrote(i0, i);
+  c.he_set(mv, ax, c.he_get(mv, ax) + (i - i0));
 }
-RS_FN void sort_max_down(ArenaState& s, const StaticData& w, int ax, uint32_t i) {
-  Edge* e = s.edges[ax];
-  const Edge cur = e[i];
+RS_FN void sort_max_down(ArenaState& s, const StaticData& w, SapCopy& c, int ax, uint32_t i) {
+  const Edge cur = c.edge(i);
   const uint32_t mv = cur.handle, i0 = i;
   while (i > 0) {
-    const Edge prev = e[i - 1];
+    const Edge prev = c.edge(i - 1);
     if (!(cur.pos < prev.pos)) break;
     if (!edge_max(prev)) {
-      if (overlap_2d(s, mv, prev.handle, ax)) remove_pair(s, w, mv, prev.handle);
-      s.handle_edges[prev.handle].min[ax] += 1;
+      if (overlap_2d(c, mv, prev.handle, ax)) remove_pair(s, w, mv, prev.handle);
+      c.he_set(prev.handle, ax, c.he_get(prev.handle, ax) + 1u);
     } else {
-      s.handle_edges[prev.handle].max[ax] += 1;
+      c.he_set(prev.handle, 3u + ax, c.he_get(prev.handle, 3u + ax) + 1u);
     }
-    e[i] = prev;
+    c.set_edge(i, prev);
     --i;
   }
   if (i == i0) return;
-  e[i] = cur;
-  s.handle_edges[mv].max[ax] = (uint16_t)(s.handle_edges[mv].max[ax] - (uint16_t)(i0 - i));
+  c.set_edge(i, cur);
+  c.wrote(i, i0);
+  c.he_set(mv, 3u + ax, c.he_get(mv, 3u + ax) - (i0 - i));
 }
 RS_FN void sap_quantize(const StaticData& w, V3 p, uint16_t is_max, uint16_t out[3]) {  // axis_sweep.cuh:16-24
   V3 sc = (p - w.sap_world_min) * w.sap_quantize;
@@ -198,17 +275,24 @@ RS_FN void sap_quantize(const StaticData& w, V3 p, uint16_t is_max, uint16_t out
   out[1] = (uint16_t)(((uint16_t)f2i_rs(cl.y) & w.sap_handle_mask) | is_max);
   out[2] = (uint16_t)(((uint16_t)f2i_rs(cl.z) & w.sap_handle_mask) | is_max);
 }
-RS_FN void update_axis(ArenaState& s, const StaticData& w, int ax, uint32_t h, uint16_t qmin, uint16_t qmax) {
-  Edge* e = s.edges[ax];
-  uint32_t mn = s.handle_edges[h].min[ax], mx = s.handle_edges[h].max[ax];
-  int32_t dmin = (int32_t)qmin - (int32_t)e[mn].pos, dmax = (int32_t)qmax - (int32_t)e[mx].pos;
+RS_FN void update_axis(ArenaState& s, const StaticData& w, SapCopy& c, int ax, uint32_t h, uint16_t qmin, uint16_t qmax) {
+  sap_axis_in(s, c, ax);
+  uint32_t mn = c.he_get(h, (uint32_t)ax), mx = c.he_get(h, 3u + ax);
+  Edge emn = c.edge(mn), emx = c.edge(mx);
+  int32_t dmin = (int32_t)qmin - (int32_t)emn.pos, dmax = (int32_t)qmax - (int32_t)emx.pos;
   if (dmin == 0 && dmax == 0) return;
-  e[mn].pos = qmin;
-  e[mx].pos = qmax;
-  if (dmin < 0) sort_min_down(s, w, ax, mn);  // expand first, then shrink
-  if (dmax > 0) sort_max_up(s, w, ax, mx);
-  if (dmin > 0) sort_min_up(s, w, ax, mn);
-  if (dmax < 0) sort_max_down(s, w, ax, mx);
+  emn.pos = qmin;
+  c.set_edge(mn, emn);
+  emx = c.edge(mx);  // mn == mx is possible in a malformed state: the second write sees the first
+  emx.pos = qmax;
+  c.set_edge(mx, emx);
+  c.wrote(mn & 63u, mn & 63u);
+  c.wrote(mx & 63u, mx & 63u);
+  if (dmin < 0) sort_min_down(s, w, c, ax, mn);  // expand first, then shrink
+  if (dmax > 0) sort_max_up(s, w, c, ax, mx);
+  if (dmin > 0) sort_min_up(s, w, c, ax, mn);
+  if (dmax < 0) sort_max_down(s, w, c, ax, mx);
+  sap_axis_out(s, c, ax);
 }
 
 // ---- dynamic ray-grid cells (grid_broadphase.cuh) ----------------------------------------------------------------------------
@@ -327,7 +411,7 @@ RS_UNROLL
   s.n_dyn_cells = kept + placed;
   s.overflow[OVF_DYN_CELLS] += total - placed;
 }
-RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb box) {  // grid_broadphase.cuh:68-84
+RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, SapCopy& c, uint32_t d, Aabb box) {  // grid_broadphase.cuh:68-84
   uint32_t h = w.dyn_handle[d];
   s.dyn_proxy[d].aabb = box;
   uint16_t qmin[3], qmax[3];
@@ -335,14 +419,18 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
   sap_quantize(w, box.max, 1, qmax);
   // the handle's six edge positions read together; an axis whose edges keep their positions is left alone, as
   // update_axis would. Sorting one axis changes only that axis's edges and edge indices, so the other axes' reads stand.
-  const HandleEdges he = s.handle_edges[h];
+  uint32_t hmin[3], hmax[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    hmin[ax] = c.he_get(h, (uint32_t)ax);
+    hmax[ax] = c.he_get(h, 3u + ax);
+  }
   uint16_t pmin[3], pmax[3];
   for (int ax = 0; ax < 3; ++ax) {
-    pmin[ax] = s.edges[ax][he.min[ax]].pos;
-    pmax[ax] = s.edges[ax][he.max[ax]].pos;
+    pmin[ax] = s.edges[ax][hmin[ax]].pos;
+    pmax[ax] = s.edges[ax][hmax[ax]].pos;
   }
   for (int ax = 0; ax < 3; ++ax)
-    if (qmin[ax] != pmin[ax] || qmax[ax] != pmax[ax]) update_axis(s, w, ax, h, qmin[ax], qmax[ax]);
+    if (qmin[ax] != pmin[ax] || qmax[ax] != pmax[ax]) update_axis(s, w, c, ax, h, qmin[ax], qmax[ax]);
   uint32_t old = s.dyn_proxy[d].cell_idx, ni[3];
   cell_indices(w, box.min, ni);
   uint32_t nidx = cell_index(w, ni);
@@ -355,15 +443,10 @@ RS_FN void grid_set_aabb(ArenaState& s, const StaticData& w, uint32_t d, Aabb bo
 }
 
 // P3a (collision_world.cuh:34-50): the dynamic bodies' swept boxes (current and predicted transform, each grown by the
-// contact breaking threshold) into the broadphase.
-// The sweep's arrays (the handles' edge indices and the three edge lists, contiguous in the state) warmed before the
-// sweep, whose accesses are chains of dependent reads.
-RS_FN void sap_warm(const ArenaState& s) {
-  static_assert(offsetof(ArenaState, edges) == offsetof(ArenaState, handle_edges) + sizeof(s.handle_edges), "contiguous");
-  warm_words<(int)((sizeof(s.handle_edges) + sizeof(s.edges)) / 4)>(&s.handle_edges[0]);
-}
+// contact breaking threshold) into the broadphase, the sweep working on its copy (SapCopy).
 RS_INLINE void phase_p3a(ArenaState& s, const StaticData& w) {
-  sap_warm(s);
+  SapCopy c;
+  sap_copy_in(s, c);
   const V3 cbt = splat(k::CONTACT_BREAKING_THRESHOLD);
   for (uint32_t d = 0; d < (uint32_t)NUM_DYN; ++d) {
     const BodyConst& bc = w.bodies[w.dyn_body[d]];
@@ -377,8 +460,9 @@ RS_INLINE void phase_p3a(ArenaState& s, const StaticData& w) {
     a2.max += cbt;
     a.min = vmin(a.min, a2.min);
     a.max = vmax(a.max, a2.max);
-    grid_set_aabb(s, w, d, a);
+    grid_set_aabb(s, w, c, d, a);
   }
+  sap_copy_out(s, c);
 }
 
 }  // namespace rs

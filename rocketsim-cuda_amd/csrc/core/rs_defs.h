@@ -78,22 +78,6 @@ inline T* rs_uniform_ptr(T* p) { return p; }
 
 namespace rs {
 
-// WORDS consecutive words of the lane's state read together and discarded: the lines a following chain of dependent reads
-// will use are then fetched in one round trip instead of one miss at a time (state touched once a tick has left the
-// caches since its last use). On the host, nothing.
-template <int WORDS>
-RS_FN void warm_words(const void* first) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const uint32_t* p = reinterpret_cast<const uint32_t*>(first);
-  uint32_t x = 0;
-  RS_UNROLL
-  for (int i = 0; i < WORDS; ++i) x ^= p[i];
-  __asm__ volatile("" ::"v"(x));
-#else
-  (void)first;
-#endif
-}
-
 RS_FN uint32_t fbits(float a) { return __builtin_bit_cast(uint32_t, a); }
 RS_FN float ffrom(uint32_t u) { return __builtin_bit_cast(float, u); }
 RS_FN uint64_t dbits(double a) { return __builtin_bit_cast(uint64_t, a); }

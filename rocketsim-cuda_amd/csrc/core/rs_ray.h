@@ -539,11 +539,6 @@ RS_FN uint32_t grid_cell_of(const StaticData& w, V3 pos) {
   z = z < nz - 1 ? z : nz - 1;
   return x * ny * nz + y * nz + z;
 }
-RS_FN Aabb proxy_box(const ArenaState& s, const StaticData& w, uint32_t h) {
-  for (int d = 0; d < NUM_DYN; ++d)
-    if (w.dyn_handle[d] == h) return s.dyn_proxy[d].aabb;
-  return w.proxies[h].aabb;
-}
 RS_FN bool ray_candidate_passes(const ProxyConst& p, const ClosestHits& hits) {  // :89-95
   return hits.ignore_body != p.client_body && (p.filter_group & hits.mask) != 0 && (hits.group & p.filter_mask) != 0;
 }
@@ -557,7 +552,12 @@ RS_FN void ray_candidate(const ArenaState& s, const StaticData& w, uint32_t h, C
 // the rays' box. The screen runs over all cells before any candidate's ray test: a ray test changes only the hits, which
 // the screen does not read, so this is the sequence of screening and testing entry by entry, with the screen's loads no
 // longer waiting behind the tests. Returns the number of candidates written to cand (at most 32).
+// A handle's box test does not depend on the cell it is met in, so every handle is tested once, up front, into a mask (the
+// world's box for a static handle, the proxy's for a dynamic one); the cells' ranges, their runs in the dynamic list
+// (bisections side by side) and their static entries (eight at a time) are then read as independent loads, and only the
+// walk in entry order, with its at-most-once rule, stays sequential.
 RS_FN int ray_screen(const ArenaState& s, const StaticData& w, const ClosestHits& hits, uint32_t* cand) {
+  static_assert(NUM_HANDLES <= 32, "one bit per handle");
   Aabb rb;
   rb.min = vmin(hits.from[0], hits.to[0]);
   rb.max = vmax(hits.from[0], hits.to[0]);
@@ -565,7 +565,7 @@ RS_FN int ray_screen(const ArenaState& s, const StaticData& w, const ClosestHits
     rb.min = vmin(rb.min, vmin(hits.from[i], hits.to[i]));
     rb.max = vmax(rb.max, vmax(hits.from[i], hits.to[i]));
   }
-  uint32_t cells[4];
+  uint32_t cells[4] = {0, 0, 0, 0};
   int nc = 0;
   for (int r = 0; r < 4; ++r) {
     uint32_t c = grid_cell_of(w, hits.from[r]);
@@ -573,31 +573,61 @@ RS_FN int ray_screen(const ArenaState& s, const StaticData& w, const ClosestHits
     for (int q = 0; q < nc; ++q) found = found || cells[q] == c;
     if (!found) cells[nc++] = c;
   }
+  uint32_t over = 0;
+  RS_UNROLL
+  for (int h = 0; h < NUM_HANDLES; ++h) over |= overlaps(rb, w.proxies[h].aabb) ? 1u << h : 0u;
+  RS_UNROLL
+  for (int d = NUM_DYN - 1; d >= 0; --d) {  // a handle's first dynamic slot gives its box
+    const uint32_t bit = 1u << w.dyn_handle[d];
+    over = overlaps(rb, s.dyn_proxy[d].aabb) ? over | bit : over & ~bit;
+  }
+  // the cell's dynamic entries: the list is sorted by cell (an invariant of the format, dump.rs:683-685), so its run
+  // is found by bisection and walked in list order, the entries a full scan would visit, in the same order
+  const uint32_t nd = s.n_dyn_cells < (uint32_t)MAX_DYN_CELLS ? s.n_dyn_cells : (uint32_t)MAX_DYN_CELLS;
+  CellRange cr[4];
+  uint32_t lo[4], hi[4];
+  RS_UNROLL
+  for (int q = 0; q < 4; ++q) {
+    cr[q] = q < nc ? w.cells[cells[q]] : CellRange{0, 0};
+    lo[q] = 0;
+    hi[q] = q < nc ? nd : 0;
+  }
+  for (bool busy = true; busy;) {
+    busy = false;
+    RS_UNROLL
+    for (int q = 0; q < 4; ++q) {
+      if (lo[q] < hi[q]) {
+        uint32_t mid = (lo[q] + hi[q]) >> 1;
+        if (s.dyn_cells[mid].cell < cells[q]) lo[q] = mid + 1;
+        else hi[q] = mid;
+        busy = true;
+      }
+    }
+  }
   uint32_t seen = 0;  // distinct handles: at most one per bit of `seen`, so at most 32 candidates
   int ncand = 0;
-  for (int ci = 0; ci < nc; ++ci) {
-    uint32_t cell = cells[ci];
-    const CellRange& cr = w.cells[cell];
-    for (uint32_t e = 0; e < cr.count; ++e) {
-      uint32_t h = w.cell_handles[cr.first + e];
-      if (seen & (1u << h)) continue;
-      seen |= 1u << h;
-      if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
+  RS_UNROLL
+  for (int q = 0; q < 4; ++q) {
+    if (q >= nc) break;
+    const uint32_t cnt = cr[q].count;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 8) {
+      uint32_t hs[8];
+      RS_UNROLL
+      for (int j = 0; j < 8; ++j) hs[j] = e0 + j < cnt ? w.cell_handles[cr[q].first + e0 + j] : 0u;
+      RS_UNROLL
+      for (int j = 0; j < 8; ++j) {
+        if (e0 + j >= cnt) break;
+        const uint32_t bit = 1u << hs[j];
+        if (seen & bit) continue;
+        seen |= bit;
+        if (over & bit) cand[ncand++] = hs[j];
+      }
     }
-    // the cell's dynamic entries: the list is sorted by cell (an invariant of the format, dump.rs:683-685), so its run
-    // is found by bisection and walked in list order, the entries a full scan would visit, in the same order
-    uint32_t lo = 0, hi = s.n_dyn_cells < (uint32_t)MAX_DYN_CELLS ? s.n_dyn_cells : (uint32_t)MAX_DYN_CELLS;
-    const uint32_t end = hi;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (s.dyn_cells[mid].cell < cell) lo = mid + 1;
-      else hi = mid;
-    }
-    for (uint32_t e = lo; e < end && s.dyn_cells[e].cell == cell; ++e) {
-      uint32_t h = s.dyn_cells[e].handle;
-      if (seen & (1u << h)) continue;
-      seen |= 1u << h;
-      if (overlaps(rb, proxy_box(s, w, h))) cand[ncand++] = h;
+    for (uint32_t e = lo[q]; e < nd && s.dyn_cells[e].cell == cells[q]; ++e) {
+      const uint32_t h = s.dyn_cells[e].handle, bit = 1u << h;
+      if (seen & bit) continue;
+      seen |= bit;
+      if (over & bit) cand[ncand++] = h;
     }
   }
   return ncand;

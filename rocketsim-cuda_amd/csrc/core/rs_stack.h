@@ -18,7 +18,8 @@ constexpr int BANK_BODIES = 8;
 constexpr int BANK_BODY_WORDS = 16;
 constexpr int BANK_ROWS = 32;
 constexpr int BANK_WORDS = BANK_BODIES * BANK_BODY_WORDS + BANK_ROWS;
-// Both live in the same per-lane LDS slice (they are used in different phases). Each wave of the work-group owns
+// Both live in the same per-lane LDS slice, as does the broadphase sweep's working copy (rs_broad.h): each is used in a
+// phase of its own (the walks in P1 and P3b, the sweep in P3a, the bank in P5). Each wave of the work-group owns
 // 64 * LANE_LDS_WORDS words, entry-major: word k of lane l of wave v is at v * 64 * LANE_LDS_WORDS + k * 64 + l. The layout
 // does not depend on the work-group size (a kernel allocates whole waves), so the device functions never read it: they
 // need no dispatch-packet pointer, which every call would otherwise carry.
