@@ -30,14 +30,17 @@ RS_HD inline size_t trav_lds_bytes(uint32_t block) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-extern __shared__ uint32_t rs_trav_lds[];
+// The first word of this wave's slices. The stepping kernels declare no static LDS (a group segment of fixed size 0, which
+// rsim_open checks), so their dynamic LDS starts at LDS address 0 and the address is formed from that. Through an extern
+// __shared__ array a non-kernel function would find the array's address per kernel, in a table indexed by a kernel id
+// that every call then passes, and each caller keeps spilled across its phase calls.
+__device__ inline uint32_t* wave_lds() {
+  const uint32_t byte = (threadIdx.x >> 6) * WAVE_LDS_WORDS * (uint32_t)sizeof(uint32_t);
+  return (uint32_t*)(__attribute__((address_space(3))) uint32_t*)byte;
+}
 // Word k of this lane's slice, and word k of lane `slot`'s slice of the same wave.
-__device__ inline uint32_t* lane_word(uint32_t k) {
-  return rs_trav_lds + ((threadIdx.x >> 6) * WAVE_LDS_WORDS + k * 64u + (threadIdx.x & 63u));
-}
-__device__ inline uint32_t* wave_lane_word(uint32_t slot, uint32_t k) {
-  return rs_trav_lds + ((threadIdx.x >> 6) * WAVE_LDS_WORDS + k * 64u + slot);
-}
+__device__ inline uint32_t* lane_word(uint32_t k) { return wave_lds() + (k * 64u + (threadIdx.x & 63u)); }
+__device__ inline uint32_t* wave_lane_word(uint32_t slot, uint32_t k) { return wave_lds() + (k * 64u + slot); }
 struct TravStack {
   uint32_t spill[TRAV_CAP - TRAV_LDS];
   int n;

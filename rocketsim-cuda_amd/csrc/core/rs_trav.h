@@ -99,7 +99,7 @@ constexpr uint32_t PT_REC = 10;  // a job record: box (6 words), the address of 
 constexpr uint32_t PT_JOB_W = 0, PT_CNT_W = PT_JOB_W + PT_REC * PT_JOBS, PT_START_W = PT_CNT_W + PT_JOBS, PT_CTRL_W = PT_START_W + PT_JOBS,
                    PT_Q_W = PT_CTRL_W + 4, PT_L_W = PT_Q_W + 4 * PT_Q, PT_WORDS = PT_L_W + 2 * PT_L;
 static_assert(PT_WORDS <= 64u * (uint32_t)LANE_LDS_WORDS, "the pooled walk fits the wave's LDS slices");
-__device__ inline uint32_t* wave_word(uint32_t j) { return rs_trav_lds + ((threadIdx.x >> 6) * WAVE_LDS_WORDS + j); }
+__device__ inline uint32_t* wave_word(uint32_t j) { return wave_lds() + j; }
 __device__ inline uint32_t* pt_q(uint32_t which, uint32_t i, uint32_t k) { return wave_word(PT_Q_W + which * 2 * PT_Q + 2 * i + k); }
 
 __device__ inline bool trav_pooled(const StaticData& w, TravBatch& tb) {

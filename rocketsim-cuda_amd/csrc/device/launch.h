@@ -42,6 +42,9 @@ hipError_t launch_sim8_ticks(hipStream_t st, const Sim8Args& p, uint32_t block, 
 // first; order: null or a permutation of the ceil(n / 64) block indices.
 hipError_t launch_run_queue(hipStream_t st, const Sim8Args& p, uint32_t decisions, uint32_t block, const uint32_t* order,
                             uint32_t* next, uint32_t workgroups);
+// The largest static LDS (group segment of fixed size) of the kernels that step arenas: their device functions address the
+// dynamic LDS from LDS address 0, which holds only while this is 0.
+hipError_t stepping_static_lds(size_t* bytes);
 // How many work-groups of `block` lanes of k_run_queue are resident at once on the current GPU (0 if unknown).
 uint32_t run_queue_capacity(uint32_t block);
 

@@ -232,6 +232,21 @@ hipError_t launch_sim8_ticks(hipStream_t st, const Sim8Args& p, uint32_t block, 
   }
   return hipSuccess;
 }
+hipError_t stepping_static_lds(size_t* bytes) {
+  const void* kernels[] = {(const void*)k_phase,        (const void*)k_step_tick,           (const void*)k_sim8<false, false>,
+                           (const void*)k_sim8<false, true>, (const void*)k_sim8<true, false>, (const void*)k_sim8<true, true>,
+                           (const void*)k_sim8_ticks<false>, (const void*)k_sim8_ticks<true>, (const void*)k_run,
+                           (const void*)k_run_queue};
+  size_t most = 0;
+  for (const void* k : kernels) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, k);
+    if (e != hipSuccess) return e;
+    most = a.sharedSizeBytes > most ? a.sharedSizeBytes : most;
+  }
+  *bytes = most;
+  return hipSuccess;
+}
 uint32_t run_queue_capacity(uint32_t block) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)

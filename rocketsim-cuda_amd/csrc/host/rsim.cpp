@@ -172,11 +172,23 @@ int rsim_open(int ordinal, rsim_device** out) {
   if (e == hipSuccess) e = hipEventCreate(&d->t1);
   // the device address of this GPU's fixed order workspace, looked up once with the GPU current
   if (e == hipSuccess) e = rs::dev::order_scratch_address(&d->order_scratch);
+  size_t static_lds = 0;
+  if (e == hipSuccess) e = rs::dev::stepping_static_lds(&static_lds);
   if (e == hipSuccess) e = hipMalloc((void**)&d->queue, 256);
-  if (e != hipSuccess) {
+  auto discard = [d]() {
     if (d->queue) (void)hipFree(d->queue);
+    if (d->t1) (void)hipEventDestroy(d->t1);
+    if (d->t0) (void)hipEventDestroy(d->t0);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
+  };
+  if (e != hipSuccess) {
+    discard();
     return hip_fail(e, "rsim_open");
+  }
+  if (static_lds != 0) {  // a build whose stepping kernels gained static LDS: their LDS addressing would overlap it
+    discard();
+    return fail(RSIM_ERR_HIP, "rsim_open: the stepping kernels declare static LDS; their dynamic LDS would not start at 0");
   }
   *out = d;
   return RSIM_OK;
