@@ -145,25 +145,32 @@ RS_FN void refresh_points(const Np& c, Manifold& m, uint32_t b0, uint32_t b1) { 
   if (m.n_points == 0) return;
   Aff ta = body_trans(c, b0), tb = body_trans(c, b1);
   const uint32_t n = m.n_points;
+  // the refreshed values stay in registers for the removal pass: a removal moves the last point into a slot the pass has
+  // already left, so point i is still the one refreshed here when the pass reaches it
+  V3 wa[MAX_POINTS] = {}, wb[MAX_POINTS] = {}, nb[MAX_POINTS] = {};
+  float dist[MAX_POINTS] = {};
   RS_UNROLL
   for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
     if (i >= n) continue;
     ManifoldPoint& p = m.pts[i];
-    p.world_on_a = xform_point(ta, p.local_a);
-    p.world_on_b = xform_point(tb, p.local_b);
-    p.distance = dot(p.world_on_a - p.world_on_b, p.normal_on_b);
+    wa[i] = xform_point(ta, p.local_a);
+    wb[i] = xform_point(tb, p.local_b);
+    nb[i] = p.normal_on_b;
+    dist[i] = dot(wa[i] - wb[i], nb[i]);
+    p.world_on_a = wa[i];
+    p.world_on_b = wb[i];
+    p.distance = dist[i];
   }
   float cbt = breaking_threshold(c, m), cbt2 = cbt * cbt;
   RS_UNROLL
   for (int i = MAX_POINTS - 1; i >= 0; --i) {  // from the last point down, over the points present at the start
     if (i >= (int)n) continue;
-    ManifoldPoint p = m.pts[i];
-    if (p.distance > cbt) {
+    if (dist[i] > cbt) {
       point_swap_remove(m, (uint32_t)i);
       continue;
     }
-    V3 proj = p.world_on_a - p.normal_on_b * p.distance;
-    V3 diff = p.world_on_b - proj;
+    V3 proj = wa[i] - nb[i] * dist[i];
+    V3 diff = wb[i] - proj;
     if (dot(diff, diff) > cbt2) point_swap_remove(m, (uint32_t)i);
   }
 }
@@ -360,9 +367,12 @@ RS_FN void add_contact(Np& c, Manifold& m, uint32_t b0, uint32_t b1, V3 normal_o
   np.world_on_b = point_world;
   np.friction = combined_friction(c, b0, b1);
   np.restitution = combined_restitution(c, b0, b1);
-  uint32_t at = add_point_no_callback(c, m, np);
+  // The callback runs on the new point before it is stored rather than on its slot after: it reads and writes only the
+  // point (never its local_a or distance, which choose the slot) and the tracker's records, so the stored point and the
+  // records are the same, and the point is written to the manifold once instead of written, read back and rewritten.
   if ((body_flags(c, b0) & k::CF_CUSTOM_MATERIAL) != 0 || (body_flags(c, b1) & k::CF_CUSTOM_MATERIAL) != 0)
-    contact_added(c, &m.pts[at], b0, b1, tri_idx);
+    contact_added(c, &np, b0, b1, tri_idx);
+  add_point_no_callback(c, m, np);
 }
 
 }  // namespace rs
