@@ -73,58 +73,77 @@ RS_FN void point_swap_remove(Manifold& m, uint32_t i) {
   m.n_points = last;
 }
 RS_FN float res_area(V3 p, V3 a, V3 b, V3 c) { return length_sq(cross(p - a, b - c)); }  // get_res
-RS_FN uint32_t replace_slot(const Manifold& m, const ManifoldPoint& np) {  // sort_cached_points (:81-117)
-  uint32_t deepest = MAX_POINTS;
-  float maxpen = np.distance;
-  const uint32_t n = m.n_points;
-  RS_UNROLL
-  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i)  // the point loops run over the four slots with a guard, so the
-    if (i < n && m.pts[i].distance < maxpen) {       // points' loads need not wait for each other
-      deepest = i;
-      maxpen = m.pts[i].distance;
-    }
-  V3 p = np.local_a;
-  const ManifoldPoint* q = m.pts;
-  float r0 = deepest == 0 ? 0.0f : res_area(p, q[1].local_a, q[3].local_a, q[2].local_a);
-  float r1 = deepest == 1 ? 0.0f : res_area(p, q[0].local_a, q[3].local_a, q[2].local_a);
-  float r2 = deepest == 2 ? 0.0f : res_area(p, q[0].local_a, q[3].local_a, q[1].local_a);
-  float r3 = deepest == 3 ? 0.0f : res_area(p, q[0].local_a, q[2].local_a, q[1].local_a);
-  uint32_t best = 0;
-  float mx = r0;
-  if (r1 > mx) { best = 1; mx = r1; }
-  if (r2 > mx) { best = 2; mx = r2; }
-  if (r3 > mx) best = 3;
-  return best;
-}
-RS_FN int cache_entry(const Np& c, const Manifold& m, const ManifoldPoint& np) {  // get_cache_entry (:120-133)
-  float cbt = breaking_threshold(c, m);
-  float shortest = cbt * cbt;
-  int nearest = -1;
-  const uint32_t n = m.n_points;
+// A manifold's slot keys held in registers while points are added to it: each point's local_a and distance, the point
+// count and the breaking threshold, which are all that choose an added point's slot. A run of additions then reads them
+// once instead of reading back, for every addition, the points the previous ones stored.
+struct PointKeys {
+  V3 la[MAX_POINTS];
+  float dist[MAX_POINTS];
+  uint32_t n;
+  float cbt;
+};
+RS_FN PointKeys point_keys(const Np& c, const Manifold& m) {
+  PointKeys pk;
+  pk.n = m.n_points;
+  pk.cbt = breaking_threshold(c, m);
   RS_UNROLL
   for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
-    if (i >= n) continue;
-    float d2 = length_sq(m.pts[i].local_a - np.local_a);
+    pk.la[i] = i < pk.n ? m.pts[i].local_a : V3{0.0f, 0.0f, 0.0f};
+    pk.dist[i] = i < pk.n ? m.pts[i].distance : 0.0f;
+  }
+  return pk;
+}
+// persistent_manifold.rs add_point_no_callback (persistent_manifold.cuh:139-163) on the keys: the nearest cached point within
+// the breaking threshold (get_cache_entry, :120-133), else the next free slot, else the replacement slot of
+// sort_cached_points (:81-117); the point is stored there and the keys updated.
+RS_FN uint32_t add_point_keyed(Manifold& m, PointKeys& pk, const ManifoldPoint& np) {
+  int at = -1;  // cache_entry
+  float shortest = pk.cbt * pk.cbt;
+  RS_UNROLL
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i) {
+    if (i >= pk.n) continue;
+    float d2 = length_sq(pk.la[i] - np.local_a);
     if (d2 < shortest) {
       shortest = d2;
-      nearest = (int)i;
+      at = (int)i;
     }
   }
-  return nearest;
-}
-RS_FN uint32_t add_point_no_callback(const Np& c, Manifold& m, const ManifoldPoint& np) {  // :139-163
-  int at = cache_entry(c, m, np);
+  uint32_t slot;
   if (at >= 0) {
-    m.pts[at] = np;
-    return (uint32_t)at;
+    slot = (uint32_t)at;
+  } else if (pk.n == (uint32_t)MAX_POINTS) {  // replace_slot
+    static_assert(MAX_POINTS == 4, "the replacement areas are written for four points");
+    uint32_t deepest = MAX_POINTS;
+    float maxpen = np.distance;
+    RS_UNROLL
+    for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i)
+      if (pk.dist[i] < maxpen) {
+        deepest = i;
+        maxpen = pk.dist[i];
+      }
+    V3 p = np.local_a;
+    float r0 = deepest == 0 ? 0.0f : res_area(p, pk.la[1], pk.la[3], pk.la[2]);
+    float r1 = deepest == 1 ? 0.0f : res_area(p, pk.la[0], pk.la[3], pk.la[2]);
+    float r2 = deepest == 2 ? 0.0f : res_area(p, pk.la[0], pk.la[3], pk.la[1]);
+    float r3 = deepest == 3 ? 0.0f : res_area(p, pk.la[0], pk.la[2], pk.la[1]);
+    uint32_t best = 0;
+    float mx = r0;
+    if (r1 > mx) { best = 1; mx = r1; }
+    if (r2 > mx) { best = 2; mx = r2; }
+    if (r3 > mx) best = 3;
+    slot = best;
+  } else {
+    slot = pk.n++;
+    m.n_points = pk.n;
   }
-  if (m.n_points == (uint32_t)MAX_POINTS) {
-    uint32_t slot = replace_slot(m, np);
-    m.pts[slot] = np;
-    return slot;
-  }
-  m.pts[m.n_points] = np;
-  return m.n_points++;
+  m.pts[slot] = np;
+  RS_UNROLL
+  for (uint32_t i = 0; i < (uint32_t)MAX_POINTS; ++i)
+    if (i == slot) {
+      pk.la[i] = np.local_a;
+      pk.dist[i] = np.distance;
+    }
+  return slot;
 }
 RS_FN float combined_friction(const Np& c, uint32_t b0, uint32_t b1) {
   float f0 = c.w.bodies[b0].friction, f1 = c.w.bodies[b1].friction;
@@ -138,7 +157,8 @@ RS_FN void merge_points(const Np& c, Manifold& m, const Manifold& o) {  // :196-
   if (m.n_points == 0) {
     for (uint32_t i = 0; i < o.n_points; ++i) m.pts[m.n_points++] = o.pts[i];
   } else {
-    for (uint32_t i = 0; i < o.n_points; ++i) add_point_no_callback(c, m, o.pts[i]);
+    PointKeys pk = point_keys(c, m);
+    for (uint32_t i = 0; i < o.n_points; ++i) add_point_keyed(m, pk, o.pts[i]);
   }
 }
 RS_FN void refresh_points(const Np& c, Manifold& m, uint32_t b0, uint32_t b1) {  // :208-213 with persistent_manifold.rs:312
@@ -355,8 +375,9 @@ RS_FN void contact_added(Np& c, ManifoldPoint* mp, uint32_t a, uint32_t b, int32
   if (tri_idx >= 0) fix_internal_edge(c, mp, b, (uint32_t)tri_idx);
 }
 // persistent_manifold.rs add_contact_point (persistent_manifold.cuh:166-192). tri_idx is the mesh triangle, or -1.
-RS_FN void add_contact(Np& c, Manifold& m, uint32_t b0, uint32_t b1, V3 normal_on_b, V3 point_world, float depth, int32_t tri_idx) {
-  if (depth > breaking_threshold(c, m)) return;
+RS_FN void add_contact(Np& c, Manifold& m, PointKeys& pk, uint32_t b0, uint32_t b1, V3 normal_on_b, V3 point_world, float depth,
+                       int32_t tri_idx) {
+  if (depth > pk.cbt) return;
   V3 pa = point_world + normal_on_b * depth;
   ManifoldPoint np = ManifoldPoint{};
   np.local_a = inv_xform(body_trans(c, b0), pa);
@@ -372,7 +393,11 @@ RS_FN void add_contact(Np& c, Manifold& m, uint32_t b0, uint32_t b1, V3 normal_o
   // records are the same, and the point is written to the manifold once instead of written, read back and rewritten.
   if ((body_flags(c, b0) & k::CF_CUSTOM_MATERIAL) != 0 || (body_flags(c, b1) & k::CF_CUSTOM_MATERIAL) != 0)
     contact_added(c, &np, b0, b1, tri_idx);
-  add_point_no_callback(c, m, np);
+  add_point_keyed(m, pk, np);
+}
+RS_FN void add_contact(Np& c, Manifold& m, uint32_t b0, uint32_t b1, V3 normal_on_b, V3 point_world, float depth, int32_t tri_idx) {
+  PointKeys pk = point_keys(c, m);
+  add_contact(c, m, pk, b0, b1, normal_on_b, point_world, depth, tri_idx);
 }
 
 }  // namespace rs

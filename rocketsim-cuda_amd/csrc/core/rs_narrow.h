@@ -259,10 +259,11 @@ RS_FN void box_box_contacts(Np& c, Obb o1, M3 r1t, Obb o2, M3 r2t, const BoxHit&
   }
   if (cnum == 0) return;
   uint32_t maxc = cnum < (uint32_t)MAX_POINTS ? cnum : (uint32_t)MAX_POINTS;
+  PointKeys pk = point_keys(c, m);
   if (cnum <= maxc) {
     for (uint32_t i = 0; i < cnum; ++i) {
       V3 pw = point[i] + o1.center;
-      add_contact(c, m, b1, b2, -hit.normal, hit.code < 4 ? pw : pw - hit.normal * dep[i], fneg(dep[i]), -1);
+      add_contact(c, m, pk, b1, b2, -hit.normal, hit.code < 4 ? pw : pw - hit.normal * dep[i], fneg(dep[i]), -1);
     }
     return;
   }
@@ -274,7 +275,7 @@ RS_FN void box_box_contacts(Np& c, Obb o1, M3 r1t, Obb o2, M3 r2t, const BoxHit&
   for (uint32_t t = 0; t < maxc; ++t) {
     uint32_t i = keep[t];
     V3 pw = point[i] + o1.center;
-    add_contact(c, m, b1, b2, -hit.normal, hit.code < 4 ? pw : pw - hit.normal * dep[i], fneg(dep[i]), -1);
+    add_contact(c, m, pk, b1, b2, -hit.normal, hit.code < 4 ? pw : pw - hit.normal * dep[i], fneg(dep[i]), -1);
   }
 }
 RS_FN bool alg_box_box(Np& c, uint32_t ba, uint32_t bb, Manifold* out) {  // :379-409
@@ -463,6 +464,8 @@ RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
   float radius = c.w.bodies[ball].sphere_margin;
   float rwt = radius + breaking_threshold(c, m), rwt2 = rwt * rwt;
   Aabb box = sphere_box(radius, ctr);
+  PointKeys pk;  // read at the first contact
+  bool keyed = false;
   const TravJob* jb = trav_find(c, TJ_BALL, ball, mesh_body);
   if (jb && !jb->in_place) {  // the walk ran ahead: its leaves in depth-first order
     // the next leaf's triangle is loaded before this one's test and contact (loads only: the same sequence of tests)
@@ -483,7 +486,10 @@ RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
       }
       V3 n, p;
       float depth;
-      if (sphere_triangle(t, ctr, radius, rwt, rwt2, &n, &p, &depth)) add_contact(c, m, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
+      if (!sphere_triangle(t, ctr, radius, rwt, rwt2, &n, &p, &depth)) continue;
+      if (!keyed) pk = point_keys(c, m);
+      keyed = true;
+      add_contact(c, m, pk, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
     }
   } else {
     BvhCursor cur;
@@ -492,8 +498,10 @@ RS_FN void sphere_mesh(Np& c, uint32_t ball, uint32_t mesh_body, Manifold& m) {
     while (cursor_next(cur, c.w, *mesh, box, &tri)) {
       V3 n, p;
       float depth;
-      if (sphere_triangle(c.w.tris[mesh->tri_first + tri], ctr, radius, rwt, rwt2, &n, &p, &depth))
-        add_contact(c, m, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
+      if (!sphere_triangle(c.w.tris[mesh->tri_first + tri], ctr, radius, rwt, rwt2, &n, &p, &depth)) continue;
+      if (!keyed) pk = point_keys(c, m);
+      keyed = true;
+      add_contact(c, m, pk, ball, mesh_body, n, p + mt.t, depth, (int32_t)tri);
     }
   }
   if (m.n_points != 0) refresh_points(c, m, ball, mesh_body);
@@ -641,6 +649,7 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
   const Tri* tris = c.w.tris + mesh->tri_first;
   Manifold& m = *out;
   manifold_init(m, car, mesh_body);
+  PointKeys pk = point_keys(c, m);
   bool has_m = false;
   const SatJob* job = nullptr;
   if (c.sb)
@@ -655,7 +664,7 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
       if (!((c.sb->ok >> t) & 1u)) continue;
       const uint32_t tri = c.sb->tri[t];
       has_m = true;  // the manifold exists from the first contact on
-      add_contact(c, m, car, mesh_body, f.mt.m * tris[tri].normal, c.sb->res[t].point_world, c.sb->res[t].distance, (int32_t)tri);
+      add_contact(c, m, pk, car, mesh_body, f.mt.m * tris[tri].normal, c.sb->res[t].point_world, c.sb->res[t].distance, (int32_t)tri);
     }
   } else {
     BvhCursor cur;
@@ -668,7 +677,7 @@ RS_FN bool car_mesh(Np& c, uint32_t car, uint32_t mesh_body, Manifold* out) {
       SatContact ct;
       if (!box_triangle_sat(f.box_t, bc.box_half_extents, bc.box_margin, q, f.maxd, &ct)) continue;
       has_m = true;
-      add_contact(c, m, car, mesh_body, f.mt.m * t.normal, ct.point_world, ct.distance, (int32_t)tri);
+      add_contact(c, m, pk, car, mesh_body, f.mt.m * t.normal, ct.point_world, ct.distance, (int32_t)tri);
     }
   }
   if (!has_m) return false;
