@@ -16,6 +16,8 @@ struct Np {
   uint32_t unimplemented;
   SatBatch* sb;
   TravBatch* tb;
+  uint32_t fold;       // event bits the records fold in during the phase, or-ed into the state's fold_events at its end
+  uint32_t n_records;  // the state's record count during the phase, written back at its end
 };
 RS_FN Aff body_trans(const Np& c, uint32_t b) {
   // Both candidates are loaded, each from its own memory (a dynamic body's transform from the arena state, a static
@@ -208,22 +210,22 @@ RS_FN void push_record(Np& c, bool swapped, uint32_t a, uint32_t b, const Manifo
   if (ua == k::USER_CAR) {
     if (ub != k::USER_BALL && ub != k::USER_CAR) {
       uint32_t car = c.w.bodies[a].user_pointer;
-      s.fold_events |= 1u << (EV_CAR_HIT_WORLD + car);
+      c.fold |= 1u << (EV_CAR_HIT_WORLD + car);
       s.fold_car_normal[car] = p.normal_on_b;
       return;
     }
   } else if (ua == k::USER_BALL && ub == k::USER_NONE) {
-    s.fold_events |= 1u << EV_BALL_HIT_WORLD;
-    if (fabs_(p.normal_on_b.z) < 0.8f) s.fold_events |= 1u << EV_BALL_WALL;
+    c.fold |= 1u << EV_BALL_HIT_WORLD;
+    if (fabs_(p.normal_on_b.z) < 0.8f) c.fold |= 1u << EV_BALL_WALL;
     return;
   } else {
     return;
   }
-  if (s.n_records >= (uint32_t)MAX_RECORDS) {
+  if (c.n_records >= (uint32_t)MAX_RECORDS) {
     s.overflow[OVF_RECORDS] += 1;
     return;
   }
-  ContactRecord& r = s.records[s.n_records++];
+  ContactRecord& r = s.records[c.n_records++];
   r.is_swap = swapped ? 1u : 0u;
   r.body_a = a;
   r.body_b = b;

@@ -817,7 +817,9 @@ RS_FN int mesh_job_of(const Np& c, uint32_t a, uint32_t b, uint32_t* body, uint3
 }
 // P3b (:230-282): the retained pairs in pair-array order. Returns the number of unimplemented pair algorithms reached.
 RS_INLINE uint32_t phase_p3b_body(ArenaState& s, const StaticData& w) {
-  Np c = {s, w, 0, nullptr, nullptr};
+  // the records' event bits and count are kept in registers and written once (nothing else reads them before P7)
+  const uint32_t n_records0 = s.n_records;
+  Np c = {s, w, 0, nullptr, nullptr, 0u, n_records0};
   uint32_t n = s.n_pairs < (uint32_t)MAX_PAIRS ? s.n_pairs : (uint32_t)MAX_PAIRS;
   TravBatch tb;
   tb.n_jobs = tb.n_leaves = tb.next = tb.leaves = 0;
@@ -844,6 +846,8 @@ RS_INLINE uint32_t phase_p3b_body(ArenaState& s, const StaticData& w) {
     Pair p = s.pairs[i];
     near_callback(c, w.proxies[p.proxy0].client_body, w.proxies[p.proxy1].client_body);
   }
+  if (c.fold != 0) s.fold_events |= c.fold;
+  if (c.n_records != n_records0) s.n_records = c.n_records;
   return c.unimplemented;
 }
 // The phase as a function of its own whose state and world pointers carry their address spaces: its accesses compile to
