@@ -66,7 +66,8 @@ typedef struct rsim_world rsim_world;   /* StaticData on the device (DeviceStati
 typedef struct rsim_states rsim_states; /* n arenas in one AoSoA-64 device buffer (DeviceStates, device.rs:41-45) */
 typedef struct rsim_words rsim_words;   /* a plain u32 device buffer (controls, results, traces: CudaSlice<u32>) */
 
-/* SimDevice::new (device.rs:53-67): opens GPU `ordinal`, creates its stream (default priority) and timing events. */
+/* SimDevice::new (device.rs:53-67): opens GPU `ordinal`, creates its stream (default priority) and timing events.
+ * RSIM_ERR_HIP also if the library's stepping kernels declare static LDS (their LDS addressing assumes none). */
 int rsim_open(int ordinal, rsim_device** dev);
 /* Releases the device. Every world, state buffer, word buffer (views included) and graph made on it must be freed
  * first: while any is open this returns RSIM_ERR_INVALID and the device stays usable (the reference's Rust ownership
