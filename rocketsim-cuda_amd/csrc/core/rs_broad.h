@@ -106,11 +106,11 @@ RS_FN void remove_pair(ArenaState& s, const StaticData& w, uint32_t ha, uint32_t
 // stores. The edge indices are copied in at the start of P3a and written back at its end for the handles whose indices
 // changed; an axis's edges are copied in when that axis's sort starts and written back over the slots the sorts wrote.
 // Indices are masked to the copy's extent (a malformed state stays within its own lane's words).
-constexpr uint32_t SAP_HE_WORDS = 3u * NUM_HANDLES;
-constexpr uint32_t SAP_WORDS = SAP_HE_WORDS + NUM_EDGES;
+constexpr uint32_t SAP_HE_WORDS = 3u * NUM_HANDLES;  // then the axis's edges, at most 64 slots (NUM_EDGES of them used)
 static_assert(sizeof(HandleEdges) == 12 && sizeof(Edge) == 4 && sizeof(((ArenaState*)nullptr)->handle_edges) == 4 * SAP_HE_WORDS,
               "HandleEdges is six u16 in three words, an Edge one word");
-static_assert(NUM_HANDLES <= 32 && SAP_HE_WORDS + 64u <= (uint32_t)LANE_LDS_WORDS, "masked indices stay in the lane's slice");
+static_assert(NUM_HANDLES <= 32 && NUM_EDGES <= 64 && SAP_HE_WORDS + 64u <= (uint32_t)LANE_LDS_WORDS,
+              "masked indices stay in the lane's slice");
 struct SapCopy {
 #if !defined(__HIP_DEVICE_COMPILE__)
   uint32_t words[SAP_HE_WORDS + 64u];
