@@ -578,15 +578,27 @@ RS_INLINE bool box_triangle_sat_body(const Aff& box_trans, V3 half, float margin
   out->distance = dist;
   return true;
 }
-// The test as a function of its own; its arguments are locals of the caller, and its pointers say so (scratch_
-// instructions rather than flat ones).
-RS_NOINL bool box_triangle_sat_in(const RS_PRIVATE_AS Aff* box_trans, V3 half, float margin, const RS_PRIVATE_AS V3* q,
-                                  float maxd, RS_PRIVATE_AS SatContact* out) {
-  return box_triangle_sat_body(*(const Aff*)box_trans, half, margin, (const V3*)q, maxd, (SatContact*)out);
+// The test as a function of its own, its inputs and result passed in registers: through pointers to the caller's
+// private memory, the caller would store them for the call and the callee read them back behind those stores (and the
+// result the other way), on every test of the pooled turns. The box transform and half extents fill the calling
+// convention's register budget for aggregates, so the triangle's corners go as nine scalars.
+struct SatResult {
+  SatContact c;
+  bool ok;
+};
+RS_NOINL SatResult box_triangle_sat_in(Aff box_trans, V3 half, float margin, float q0x, float q0y, float q0z, float q1x,
+                                       float q1y, float q1z, float q2x, float q2y, float q2z, float maxd) {
+  const V3 q[3] = {v3(q0x, q0y, q0z), v3(q1x, q1y, q1z), v3(q2x, q2y, q2z)};
+  SatResult r;
+  r.c = SatContact{};
+  r.ok = box_triangle_sat_body(box_trans, half, margin, q, maxd, &r.c);
+  return r;
 }
 RS_FN bool box_triangle_sat(const Aff& box_trans, V3 half, float margin, const V3 q[3], float maxd, SatContact* out) {
-  return box_triangle_sat_in((const RS_PRIVATE_AS Aff*)&box_trans, half, margin, (const RS_PRIVATE_AS V3*)q, maxd,
-                             (RS_PRIVATE_AS SatContact*)out);
+  const SatResult r = box_triangle_sat_in(box_trans, half, margin, q[0].x, q[0].y, q[0].z, q[1].x, q[1].y, q[1].z, q[2].x,
+                                          q[2].y, q[2].z, maxd);
+  if (r.ok) *out = r.c;
+  return r.ok;
 }
 
 }  // namespace rs
