@@ -386,6 +386,9 @@ def main():
     ap.add_argument("--tune-decisions", type=int, default=25, help="decisions the autotuner measures each candidate on")
     ap.add_argument("--no-tune", action="store_true",
                     help="take the schedule recorded in profiles/autotune_contact65536.json instead of tuning now")
+    ap.add_argument("--schedule", default=None, metavar="KIND:BLOCK",
+                    help="run the configured decisions on this schedule (run:64, queue:64 ...: all decisions in one "
+                         "launch) instead of tuning or the record; profiles use it to name the kernel they measure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cost", action="store_true", help="skip the learned cost-model legs")
     ap.add_argument("--no-configs", action="store_true", help="skip the 16,384-arena open-play and 131,072-arena scale legs")
@@ -438,7 +441,15 @@ def main():
     del d_res
 
     # the autotuner on the first decisions of this batch; its choice is the configured schedule
-    if args.no_tune:
+    if args.schedule:
+        kind, _, blk = args.schedule.partition(":")
+        if kind not in ("run", "queue") or not blk.isdigit():
+            sys.exit("bench.py: --schedule takes run:BLOCK or queue:BLOCK")
+        choice = {"kind": kind, "block": int(blk), "decisions_per_launch": K}
+        if kind == "queue":
+            choice["order"] = "index"
+        tune_rec, tuned_on = None, K
+    elif args.no_tune:
         with open(RECORD_FILE) as f:
             rec = json.load(f)
         tune_rec, choice, tuned_on = None, rec["choice"], rec["workload"]["decisions"]
@@ -478,10 +489,12 @@ def main():
     value = stepped / wall
     per_launch_s = run_ms / 1000.0 / launches
     achieved = B_TICK * (stepped / launches) / per_launch_s
-    traffic = None
+    traffic = None  # the profiled HBM bytes of one launch, when they were measured on this kernel and launch size
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("kernel") == kernel and prof.get("arena_ticks_per_launch") == stepped / launches:
+            traffic = prof.get("bytes_per_launch")
 
     cost = None
     model = None
@@ -543,7 +556,8 @@ def main():
         "plain_per_launch": {"arena_ticks_per_s": stepped / plain_wall, "ms_per_decision": plain_wall * 1000.0 / K,
                              "kernel": "k_sim8", "kernel_ms_avg": plain_ms / K, "block": args.block,
                              "roofline_frac": B_TICK * (stepped / K) / (plain_ms / 1000.0 / K) / HBM_PEAK},
-        "autotune": {"record": os.path.relpath(RECORD_FILE, ROOT), "choice": choice} if tune_rec is None else {
+        "autotune": {"given": args.schedule, "choice": choice} if args.schedule else
+                    {"record": os.path.relpath(RECORD_FILE, ROOT), "choice": choice} if tune_rec is None else {
             "workload": tune_rec["workload"], "all_identical": tune_rec["all_identical"], "choice": tune_rec["choice"],
             "candidates": [{"label": c["label"], "ms_per_decision": c["ms_per_decision"], "identical": c["identical"]}
                            for c in tune_rec["candidates"]]},
