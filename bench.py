@@ -489,12 +489,14 @@ def main():
     value = stepped / wall
     per_launch_s = run_ms / 1000.0 / launches
     achieved = B_TICK * (stepped / launches) / per_launch_s
-    traffic = None  # the profiled HBM bytes of one launch, when they were measured on this kernel and launch size
+    traffic = None  # the profiled HBM bytes of one launch, when they were measured on this kernel, block and launch size
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            prof = json.load(f)
-        if prof.get("kernel") == kernel and prof.get("arena_ticks_per_launch") == stepped / launches:
-            traffic = prof.get("bytes_per_launch")
+            profs = json.load(f)
+        for prof in profs if isinstance(profs, list) else [profs]:
+            if (prof.get("kernel") == kernel and prof.get("block") == choice.get("block") and
+                    prof.get("arena_ticks_per_launch") == stepped / launches):
+                traffic = prof.get("bytes_per_launch")
 
     cost = None
     model = None
